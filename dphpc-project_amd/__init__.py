@@ -1,0 +1,275 @@
+"""gpumpi for MI355X: Python bindings of libgpumpi.so (ctypes over the C ABI of
+include/gpumpi/gpumpi.h).
+
+The product path is the HIP library; this module only marshals arguments.  Loading
+fails loudly when libgpumpi.so is missing -- there is no CPU fallback.
+
+Device buffers are passed as integer device addresses (for example
+``torch.Tensor.data_ptr()``) and streams as integer hipStream_t handles.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpumpi.so")
+
+_lib = None
+
+# gpumpi_status
+OK, E_ARG, E_HIP, E_RESIDENCY, E_PROGRAM, E_DEVICE, E_NOMEM = 0, -1, -2, -3, -4, -5, -6
+
+# MPI_Datatype enum order (include/gpumpi/mpi.h)
+DATATYPES = [
+    "MPI_CHAR", "MPI_SHORT", "MPI_INT", "MPI_LONG", "MPI_LONG_LONG_INT", "MPI_LONG_LONG",
+    "MPI_SIGNED_CHAR", "MPI_UNSIGNED_CHAR", "MPI_UNSIGNED_SHORT", "MPI_UNSIGNED",
+    "MPI_UNSIGNED_LONG", "MPI_UNSIGNED_LONG_LONG", "MPI_FLOAT", "MPI_DOUBLE",
+    "MPI_LONG_DOUBLE", "MPI_WCHAR", "MPI_C_BOOL", "MPI_INT8_T", "MPI_INT16_T", "MPI_INT32_T",
+    "MPI_INT64_T", "MPI_UINT8_T", "MPI_UINT16_T", "MPI_UINT32_T", "MPI_UINT64_T",
+    "MPI_C_COMPLEX", "MPI_C_FLOAT_COMPLEX", "MPI_C_DOUBLE_COMPLEX",
+    "MPI_C_LONG_DOUBLE_COMPLEX", "MPI_BYTE",
+]
+DT = {n: i for i, n in enumerate(DATATYPES)}
+
+
+class GpumpiError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("num_ranks", ctypes.c_int32), ("ranks_per_block", ctypes.c_int32),
+        ("ring_slots", ctypes.c_int32), ("max_requests", ctypes.c_int32),
+        ("max_unexpected", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("heap_bytes", ctypes.c_uint64), ("stack_bytes", ctypes.c_uint64),
+        ("timeout_s", ctypes.c_double),
+    ]
+
+
+class A2avTune(ctypes.Structure):
+    _fields_ = [
+        ("block_threads", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+        ("lines_per_lane", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Exported C-ABI symbols and their ctypes signatures (kept in the order of gpumpi.h).
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_SIGNATURES = {
+    "gpumpi_config_default": (None, [ctypes.POINTER(Config)]),
+    "gpumpi_runtime_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
+    "gpumpi_runtime_destroy": (None, [_P]),
+    "gpumpi_run": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                  _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_run_async": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t, _P]),
+    "gpumpi_launch": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                     _P, ctypes.c_size_t, ctypes.POINTER(Config)]),
+    "gpumpi_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+    "gpumpi_last_error": (ctypes.c_char_p, [_P]),
+    "gpumpi_last_device_error": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "gpumpi_program_count": (ctypes.c_int, []),
+    "gpumpi_program_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "gpumpi_heap_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "gpumpi_device_info": (ctypes.c_int, [ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "gpumpi_a2av_plan_create": (ctypes.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _P,
+                                               ctypes.POINTER(A2avTune), ctypes.POINTER(_P)]),
+    "gpumpi_a2av_plan_destroy": (None, [_P]),
+    "gpumpi_a2av_execute": (ctypes.c_int, [_P, _P]),
+    "gpumpi_a2av_graph_replay": (ctypes.c_int, [_P, _I32, _P]),
+    "gpumpi_a2av_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(A2avTune), ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_a2av_get_tune": (ctypes.c_int, [_P, ctypes.POINTER(A2avTune), ctypes.POINTER(_I32)]),
+    "gpumpi_a2av_payload_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+}
+
+
+def build(verbose=False):
+    from importlib import util
+    spec = util.spec_from_file_location("gpumpi_build", os.path.join(HERE, "build.py"))
+    mod = util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.build(verbose=verbose)
+
+
+def load_library():
+    """Load libgpumpi.so; raises GpumpiError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GpumpiError(f"{LIB_PATH} is missing: run __graft_entry__.build() first "
+                          "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def program_names():
+    lib = load_library()
+    return [lib.gpumpi_program_name(i).decode() for i in range(lib.gpumpi_program_count())]
+
+
+def device_info():
+    lib = load_library()
+    cus, khz, waves = _I32(), _I32(), _I32()
+    rc = lib.gpumpi_device_info(ctypes.byref(cus), ctypes.byref(khz), ctypes.byref(waves))
+    if rc != OK:
+        raise GpumpiError("gpumpi_device_info failed")
+    return {"cus": cus.value, "wallclock_khz": khz.value, "max_waves_per_cu": waves.value}
+
+
+def default_config(**kw):
+    lib = load_library()
+    c = Config()
+    lib.gpumpi_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _argv(args):
+    arr = (ctypes.c_char_p * (len(args) + 1))()
+    for i, a in enumerate(args):
+        arr[i] = a.encode() if isinstance(a, str) else a
+    arr[len(args)] = None
+    return arr
+
+
+class Runtime:
+    """A rank-grid runtime (gpumpi_runtime_create) running built-in rank programs."""
+
+    def __init__(self, num_ranks, **cfg):
+        self.lib = load_library()
+        self.cfg = default_config(num_ranks=num_ranks, **cfg)
+        h = _P()
+        rc = self.lib.gpumpi_runtime_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_runtime_create failed ({rc}): "
+                              f"{self.lib.gpumpi_last_error(None).decode()}")
+        self.h = h
+
+    def run(self, program, args=None, argv=(), stream=0, check=True):
+        """Run `program` on every rank; args is a ctypes.Structure (the argument block).
+        Returns (status, elapsed_ms)."""
+        ms = ctypes.c_double(0.0)
+        blob = ctypes.byref(args) if args is not None else None
+        size = ctypes.sizeof(args) if args is not None else 0
+        av = _argv(list(argv)) if argv else None
+        rc = self.lib.gpumpi_run(self.h, program.encode(), len(argv), av, blob, size,
+                                 ctypes.c_void_p(stream) if stream else None, ctypes.byref(ms))
+        if check and rc != 0:
+            raise GpumpiError(f"{program}: status {rc}: {self.last_error()}")
+        return rc, ms.value
+
+    def run_async(self, program, args=None, stream=0):
+        blob = ctypes.byref(args) if args is not None else None
+        size = ctypes.sizeof(args) if args is not None else 0
+        rc = self.lib.gpumpi_run_async(self.h, program.encode(), blob, size,
+                                       ctypes.c_void_p(stream) if stream else None)
+        if rc != 0:
+            raise GpumpiError(f"{program}: status {rc}: {self.last_error()}")
+
+    def last_error(self):
+        return self.lib.gpumpi_last_error(self.h).decode()
+
+    def device_error(self):
+        code, rank = _I32(), _I32()
+        self.lib.gpumpi_last_device_error(self.h, ctypes.byref(code), ctypes.byref(rank))
+        return code.value, rank.value
+
+    def heap_stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self.lib.gpumpi_heap_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"in_use": a.value, "peak": b.value, "capacity": c.value}
+
+    def close(self):
+        if self.h:
+            self.lib.gpumpi_runtime_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class A2avPlan:
+    """Batched persistent-kernel Alltoallv step (gpumpi_a2av_*)."""
+
+    def __init__(self, P, elem_bytes, sendbufs_ptr, scounts_ptr, sdispls_ptr, recvbufs_ptr,
+                 rcounts_ptr, rdispls_ptr, tune=None):
+        self.lib = load_library()
+        t = A2avTune(0, 0, 0, -1) if tune is None else A2avTune(*tune)
+        h = _P()
+        rc = self.lib.gpumpi_a2av_plan_create(P, elem_bytes, sendbufs_ptr, scounts_ptr,
+                                              sdispls_ptr, recvbufs_ptr, rcounts_ptr,
+                                              rdispls_ptr, ctypes.byref(t), ctypes.byref(h))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_a2av_plan_create failed ({rc})")
+        self.h = h
+
+    def execute(self, stream=0):
+        rc = self.lib.gpumpi_a2av_execute(self.h, ctypes.c_void_p(stream) if stream else None)
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_a2av_execute failed ({rc})")
+
+    def graph_replay(self, steps, stream=0):
+        rc = self.lib.gpumpi_a2av_graph_replay(self.h, steps, ctypes.c_void_p(stream) if stream else None)
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_a2av_graph_replay failed ({rc})")
+
+    def autotune(self, stream=0):
+        t = A2avTune()
+        us = ctypes.c_double()
+        rc = self.lib.gpumpi_a2av_autotune(self.h, ctypes.c_void_p(stream) if stream else None,
+                                           ctypes.byref(t), ctypes.byref(us))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_a2av_autotune failed ({rc})")
+        return t.as_dict(), us.value
+
+    def tune(self):
+        t = A2avTune()
+        g = _I32()
+        self.lib.gpumpi_a2av_get_tune(self.h, ctypes.byref(t), ctypes.byref(g))
+        d = t.as_dict()
+        d["grid"] = g.value
+        return d
+
+    def payload_bytes(self):
+        b = ctypes.c_int64()
+        rc = self.lib.gpumpi_a2av_payload_bytes(self.h, ctypes.byref(b))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_a2av step reported a device error ({rc})")
+        return b.value
+
+    def close(self):
+        if self.h:
+            self.lib.gpumpi_a2av_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+from . import programs  # noqa: E402,F401  (argument blocks of the rank programs)

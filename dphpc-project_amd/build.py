@@ -1,0 +1,55 @@
+"""Build recipe of libgpumpi.so (gfx950) -- compiled in-tree with hipcc.
+
+The device runtime is header-only and is compiled together with the built-in rank
+programs and the host C ABI as one translation unit (csrc/gpumpi.hip).  Flags:
+  -ffp-contract=off          every FP expression rounds as written (bit-exact FP64
+                             reductions and Monte-Carlo hit tests against the oracle)
+  -packed-fp32-ops disabled  no v_pk_*_f32 instructions are generated
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "gpumpi.hip")
+LIB = os.path.join(HERE, "libgpumpi.so")
+ARCH = os.environ.get("GPUMPI_ARCH", "gfx950")
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found: the gpumpi HIP library cannot be built")
+
+
+def compile_cmd(out):
+    return [
+        hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+        "-ffp-contract=off", "-Wno-unused-value",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-o", out, SRC,
+    ]
+
+
+def build(verbose=False):
+    """Compile libgpumpi.so (always recompiles); atomically replaces the old library."""
+    fd, tmp = tempfile.mkstemp(prefix="libgpumpi.", suffix=".so.tmp", dir=HERE)
+    os.close(fd)
+    cmd = compile_cmd(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        os.unlink(tmp)
+        sys.stderr.write(res.stdout)
+        raise RuntimeError("hipcc failed to build libgpumpi.so")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,824 @@
+// libgpumpi.so -- the gfx950 device MPI runtime, its built-in rank programs and the host
+// C ABI of include/gpumpi/gpumpi.h, compiled as one translation unit (the device runtime
+// is header-only and is inlined into every rank program).
+#include "runtime/mpi_api.h"
+#include "runtime/batched.h"
+#include "programs/reference_tests.h"
+#include "programs/workloads.h"
+#include "programs/bucket_sort.h"
+#include "programs/serve.h"
+#include "../../include/gpumpi/gpumpi.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace gpumpi {
+
+// ------------------------------------------------------------------ rank grid kernel
+// Reference: __gpu_main_caller (gpu_main/main.cu:103-131) -- one CUDA thread per rank.
+// Here one wavefront per rank; blockDim.x = 64 * ranks_per_block.
+template <int (*MAIN)(int, char**)>
+__global__ void __launch_bounds__(256) rank_grid_kernel(Runtime* R, int argc, char** argv) {
+    s_rt = R;
+    const int rank = world_rank();
+    if (rank >= R->nranks) return;
+    int rc = MAIN(argc, argv);
+    if (rc != 0 && lane() == 0)
+        __hip_atomic_fetch_or(&R->exit_code, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    heap_flush_cache(&R->heap, &my_state()->cache);
+}
+
+struct InitBases {
+    int32_t* posted;
+    Envelope* unexp;
+    Request* reqs;
+    int32_t* free_stack;
+    char* scratch;
+};
+
+__global__ void init_runtime_kernel(Runtime* R, InitBases b) {
+    const int P = R->nranks;
+    const uint64_t ring = (uint64_t)R->ring_slots;
+    const uint64_t mr = (uint64_t)R->max_requests;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = tid; i < (uint64_t)P * ring; i += stride) {
+        R->rings[i].seq = (uint32_t)(i % ring);
+        R->rings[i].ctx = 0;
+    }
+    for (uint64_t r = tid; r < (uint64_t)P; r += stride) {
+        RankState& s = R->ranks[r];
+        s.world_rank = (int32_t)r;
+        s.n_posted = 0;
+        s.n_unexp = 0;
+        s.n_free = (int32_t)mr;
+        s.ring_head = 0;
+        s.posted = b.posted + r * mr;
+        s.unexp = b.unexp + r * (uint64_t)R->max_unexpected;
+        s.reqs = b.reqs + r * mr;
+        s.free_stack = b.free_stack + r * mr;
+        s.scratch = b.scratch + r * kScratchBytes;
+        s.initialized = 0;
+        s.finalized = 0;
+        for (int c = 0; c < kHeapSmallClasses; c++) s.cache.n[c] = 0;
+        R->mailbox[r].tail = 0;
+        R->world->slots[r].seq = 0;
+        R->world->seqs[r] = 0;
+    }
+    for (uint64_t i = tid; i < (uint64_t)P * mr; i += stride) {
+        uint64_t k = i % mr;
+        b.free_stack[i] = (int32_t)(mr - 1 - k);
+        b.reqs[i].state = kReqFree;
+        b.reqs[i].index = (int32_t)k;
+    }
+    for (uint64_t pg = tid; pg < R->heap.npages; pg += stride) R->heap.page_class[pg] = 0xFF;
+    if (tid < (uint64_t)kHeapClasses) R->heap.free_head[tid] = kHeapEmpty;
+    if (tid == 0) {
+        R->heap.page_bump = 0;
+        R->heap.bytes_in_use = 0;
+        R->heap.peak_in_use = 0;
+        R->heap.alloc_count = 0;
+        R->heap.refill_count = 0;
+        R->world->arrive.value = 0;
+        R->world->gen.value = 0;
+        R->next_ctx = 2;
+        R->abort_flag = 0;
+        R->exit_code = 0;
+        R->err_lock = 0;
+        R->err.code = 0;
+        R->err.rank = -1;
+    }
+}
+
+__global__ void clear_status_kernel(Runtime* R) {
+    if (threadIdx.x == 0) {
+        R->abort_flag = 0;
+        R->exit_code = 0;
+        R->err_lock = 0;
+        R->err.code = 0;
+        R->err.rank = -1;
+        R->err.msg[0] = 0;
+    }
+}
+
+struct ProgramEntry {
+    const char* name;
+    const void* kernel;
+};
+
+#define GPUMPI_PROGRAM(name, fn) {name, (const void*)&rank_grid_kernel<fn>}
+static const ProgramEntry kPrograms[] = {
+    GPUMPI_PROGRAM("t_single_int", progs::t_single_int),
+    GPUMPI_PROGRAM("t_array", progs::t_array),
+    GPUMPI_PROGRAM("t_sendrecv", progs::t_sendrecv),
+    GPUMPI_PROGRAM("t_repeat", progs::t_repeat),
+    GPUMPI_PROGRAM("t_flood", progs::t_flood),
+    GPUMPI_PROGRAM("t_p2p_all_to_all", progs::t_p2p_all_to_all),
+    GPUMPI_PROGRAM("t_reduce_prod_int16", progs::t_reduce_prod_int16),
+    GPUMPI_PROGRAM("t_allreduce_sum_double", progs::t_allreduce_sum_double),
+    GPUMPI_PROGRAM("t_allgather", progs::t_allgather),
+    GPUMPI_PROGRAM("t_scatter", progs::t_scatter),
+    GPUMPI_PROGRAM("t_scatterv", progs::t_scatterv),
+    GPUMPI_PROGRAM("t_comm_rank", progs::t_comm_rank),
+    GPUMPI_PROGRAM("t_comm_dup", progs::t_comm_dup),
+    GPUMPI_PROGRAM("t_comm_split", progs::t_comm_split),
+    GPUMPI_PROGRAM("mcpi", progs::w_mcpi),
+    GPUMPI_PROGRAM("p2p", progs::w_p2p),
+    GPUMPI_PROGRAM("coll", progs::w_coll),
+    GPUMPI_PROGRAM("churn", progs::w_churn),
+    GPUMPI_PROGRAM("exchange", progs::w_exchange),
+    GPUMPI_PROGRAM("bucket_sort", progs::w_bucket_sort),
+    GPUMPI_PROGRAM("npb_is", progs::w_npb_is),
+    GPUMPI_PROGRAM("particles", progs::w_particles),
+    GPUMPI_PROGRAM("job", progs::w_job),
+};
+#undef GPUMPI_PROGRAM
+constexpr int kNumPrograms = (int)(sizeof(kPrograms) / sizeof(kPrograms[0]));
+
+}  // namespace gpumpi
+
+using namespace gpumpi;
+
+// ==================================================================== host side
+struct gpumpi_runtime {
+    gpumpi_config cfg;
+    int cus = 0;
+    int wallclock_khz = 0;
+    Runtime h{};
+    Runtime* d_rt = nullptr;
+    InitBases bases{};
+    std::vector<void*> allocs;
+    void* d_args = nullptr;
+    size_t args_cap = 0;
+    hipStream_t own_stream = nullptr;
+    std::string err;
+    bool needs_reset = false;
+};
+
+static thread_local std::string g_host_error;
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            set_err(rt, std::string(#expr) + ": " + hipGetErrorString(e_));                \
+            return GPUMPI_E_HIP;                                                           \
+        }                                                                                  \
+    } while (0)
+
+static void set_err(gpumpi_runtime* rt, const std::string& s) {
+    if (rt) rt->err = s;
+    g_host_error = s;
+}
+
+static int dev_alloc(gpumpi_runtime* rt, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        set_err(rt, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return GPUMPI_E_NOMEM;
+    }
+    rt->allocs.push_back(*p);
+    return GPUMPI_OK;
+}
+
+extern "C" {
+
+void gpumpi_config_default(gpumpi_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->num_ranks = 1;
+    c->ranks_per_block = 1;
+    c->ring_slots = 256;
+    c->max_requests = 1024;
+    c->max_unexpected = 1024;
+    c->heap_bytes = 1ull << 30;
+    c->timeout_s = 20.0;
+}
+
+int gpumpi_device_info(int32_t* num_cus, int32_t* wallclock_khz, int32_t* max_waves_per_cu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return GPUMPI_E_HIP;
+    int v = 0;
+    if (num_cus) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return GPUMPI_E_HIP;
+        *num_cus = v;
+    }
+    if (wallclock_khz) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+            return GPUMPI_E_HIP;
+        *wallclock_khz = v;
+    }
+    if (max_waves_per_cu) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) != hipSuccess)
+            return GPUMPI_E_HIP;
+        *max_waves_per_cu = v / 64;
+    }
+    return GPUMPI_OK;
+}
+
+static int runtime_init_device(gpumpi_runtime* rt, hipStream_t s) {
+    hipLaunchKernelGGL(init_runtime_kernel, dim3(1024), dim3(256), 0, s, rt->d_rt, rt->bases);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    rt->needs_reset = false;
+    return GPUMPI_OK;
+}
+
+int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
+    if (!out) return GPUMPI_E_ARG;
+    *out = nullptr;
+    gpumpi_config cfg;
+    gpumpi_config_default(&cfg);
+    if (cfg_in) {
+        cfg = *cfg_in;
+        if (cfg.ranks_per_block <= 0) cfg.ranks_per_block = 1;
+        if (cfg.ring_slots <= 0) cfg.ring_slots = 256;
+        if (cfg.max_requests <= 0) cfg.max_requests = 1024;
+        if (cfg.max_unexpected <= 0) cfg.max_unexpected = 1024;
+        if (cfg.heap_bytes == 0) cfg.heap_bytes = 1ull << 30;
+        if (cfg.timeout_s <= 0) cfg.timeout_s = 20.0;
+    }
+    if (cfg.num_ranks <= 0 || cfg.ranks_per_block > 4) {
+        g_host_error = "invalid num_ranks or ranks_per_block";
+        return GPUMPI_E_ARG;
+    }
+    gpumpi_runtime* rt = new gpumpi_runtime();
+    rt->cfg = cfg;
+    int32_t cus = 0, khz = 0;
+    if (gpumpi_device_info(&cus, &khz, nullptr) != GPUMPI_OK || khz <= 0) {
+        set_err(rt, "cannot query the HIP device");
+        delete rt;
+        return GPUMPI_E_HIP;
+    }
+    rt->cus = cus;
+    rt->wallclock_khz = khz;
+    if (cfg.stack_bytes) hipDeviceSetLimit(hipLimitStackSize, cfg.stack_bytes);
+
+    const uint64_t P = (uint64_t)cfg.num_ranks;
+    Runtime& h = rt->h;
+    std::memset(&h, 0, sizeof(h));
+    h.nranks = cfg.num_ranks;
+    h.ranks_per_block = cfg.ranks_per_block;
+    h.ring_slots = cfg.ring_slots;
+    h.max_requests = cfg.max_requests;
+    h.max_unexpected = cfg.max_unexpected;
+    h.timeout_ticks = (uint64_t)(cfg.timeout_s * (double)khz * 1000.0);
+    h.wtick = 1.0 / ((double)khz * 1000.0);
+    h.next_ctx = 2;
+    int rc = GPUMPI_OK;
+    void* p = nullptr;
+#define ALLOC(field, type, count)                                                     \
+    if ((rc = dev_alloc(rt, &p, sizeof(type) * (size_t)(count))) != GPUMPI_OK) {      \
+        gpumpi_runtime_destroy(rt);                                                   \
+        return rc;                                                                    \
+    }                                                                                 \
+    field = (type*)p;
+    ALLOC(rt->d_rt, Runtime, 1)
+    ALLOC(h.mailbox, Mailbox, P)
+    ALLOC(h.rings, Envelope, P * cfg.ring_slots)
+    ALLOC(h.ranks, RankState, P)
+    ALLOC(rt->bases.posted, int32_t, P * cfg.max_requests)
+    ALLOC(rt->bases.unexp, Envelope, P * cfg.max_unexpected)
+    ALLOC(rt->bases.reqs, Request, P * cfg.max_requests)
+    ALLOC(rt->bases.free_stack, int32_t, P * cfg.max_requests)
+    ALLOC(rt->bases.scratch, char, P * kScratchBytes)
+    // heap arena: whole 64 KiB pages
+    uint64_t heap = (cfg.heap_bytes + kHeapPage - 1) & ~(kHeapPage - 1);
+    ALLOC(h.heap.base, char, heap)
+    h.heap.bytes = heap;
+    h.heap.npages = heap >> kHeapPageShift;
+    ALLOC(h.heap.page_class, uint8_t, h.heap.npages)
+    // COMM_WORLD (reference: communicator.cu:52-62, context id 1) and the groups
+    CommImpl world{};
+    CollSlot* slots;
+    uint64_t* seqs;
+    ALLOC(slots, CollSlot, P)
+    ALLOC(seqs, uint64_t, P)
+    GroupImpl* groups;
+    ALLOC(groups, GroupImpl, 2)
+    CommImpl* d_world;
+    ALLOC(d_world, CommImpl, 1)
+#undef ALLOC
+    GroupImpl hg[2];
+    hg[0].size = cfg.num_ranks;
+    hg[0].refcount = -1;
+    hg[0].ranks = nullptr;  // identity
+    hg[1].size = 0;
+    hg[1].refcount = -1;
+    hg[1].ranks = nullptr;
+    world.ctx = 1;
+    world.size = cfg.num_ranks;
+    world.refcount = -1;
+    world.is_world = 1;
+    world.world_ranks = nullptr;
+    world.inv = nullptr;
+    world.slots = slots;
+    world.seqs = seqs;
+    world.group = groups;
+    h.world = d_world;
+    h.null_comm = nullptr;
+    h.world_group = groups;
+    h.empty_group = groups + 1;
+    if (hipMemcpy(groups, hg, sizeof(hg), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_world, &world, sizeof(world), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(rt->d_rt, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+        set_err(rt, "hipMemcpy of the runtime state failed");
+        gpumpi_runtime_destroy(rt);
+        return GPUMPI_E_HIP;
+    }
+    if (hipStreamCreate(&rt->own_stream) != hipSuccess) {
+        set_err(rt, "hipStreamCreate failed");
+        gpumpi_runtime_destroy(rt);
+        return GPUMPI_E_HIP;
+    }
+    rc = runtime_init_device(rt, rt->own_stream);
+    if (rc != GPUMPI_OK) {
+        gpumpi_runtime_destroy(rt);
+        return rc;
+    }
+    *out = rt;
+    return GPUMPI_OK;
+}
+
+void gpumpi_runtime_destroy(gpumpi_runtime* rt) {
+    if (!rt) return;
+    hipDeviceSynchronize();
+    for (void* p : rt->allocs) hipFree(p);
+    if (rt->d_args) hipFree(rt->d_args);
+    if (rt->own_stream) hipStreamDestroy(rt->own_stream);
+    delete rt;
+}
+
+const char* gpumpi_last_error(gpumpi_runtime* rt) {
+    return rt ? rt->err.c_str() : g_host_error.c_str();
+}
+
+int gpumpi_program_count(void) { return kNumPrograms; }
+const char* gpumpi_program_name(int i) {
+    return (i >= 0 && i < kNumPrograms) ? kPrograms[i].name : nullptr;
+}
+
+static const ProgramEntry* find_program(const char* name) {
+    if (!name) return nullptr;
+    for (int i = 0; i < kNumPrograms; i++)
+        if (std::strcmp(kPrograms[i].name, name) == 0) return &kPrograms[i];
+    return nullptr;
+}
+
+static int stage_args(gpumpi_runtime* rt, const void* args, size_t bytes, hipStream_t s) {
+    if (bytes > rt->args_cap) {
+        if (rt->d_args) hipFree(rt->d_args);
+        rt->d_args = nullptr;
+        size_t cap = bytes < 4096 ? 4096 : bytes;
+        HIP_TRY(hipMalloc(&rt->d_args, cap));
+        rt->args_cap = cap;
+    }
+    if (bytes) HIP_TRY(hipMemcpyAsync(rt->d_args, args, bytes, hipMemcpyHostToDevice, s));
+    const void* dp = rt->d_args;
+    HIP_TRY(hipMemcpyAsync((char*)rt->d_rt + offsetof(Runtime, user_args), &dp, sizeof(dp),
+                           hipMemcpyHostToDevice, s));
+    return GPUMPI_OK;
+}
+
+static int launch_program(gpumpi_runtime* rt, const ProgramEntry* pe, int argc, char** d_argv,
+                          hipStream_t s) {
+    const int rpb = rt->cfg.ranks_per_block;
+    const int block = 64 * rpb;
+    const int grid = (rt->cfg.num_ranks + rpb - 1) / rpb;
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pe->kernel, block, 0));
+    // All ranks must be resident at once (spin-waits on each other), as the reference
+    // requires of its grid (gpu_main/main.cu:189-209).
+    if ((int64_t)per_cu * rt->cus < grid) {
+        set_err(rt, "rank grid of " + std::to_string(grid) + " workgroups exceeds the " +
+                        std::to_string((int64_t)per_cu * rt->cus) + " that can be co-resident");
+        return GPUMPI_E_RESIDENCY;
+    }
+    Runtime* d = rt->d_rt;
+    void* kargs[] = {(void*)&d, (void*)&argc, (void*)&d_argv};
+    HIP_TRY(hipLaunchKernel(pe->kernel, dim3(grid), dim3(block), kargs, 0, s));
+    return GPUMPI_OK;
+}
+
+int gpumpi_run_async(gpumpi_runtime* rt, const char* program, const void* args,
+                     size_t args_bytes, void* stream) {
+    if (!rt) return GPUMPI_E_ARG;
+    const ProgramEntry* pe = find_program(program);
+    if (!pe) {
+        set_err(rt, std::string("unknown rank program: ") + (program ? program : "(null)"));
+        return GPUMPI_E_PROGRAM;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : rt->own_stream;
+    int rc = stage_args(rt, args, args_bytes, s);
+    if (rc != GPUMPI_OK) return rc;
+    return launch_program(rt, pe, 0, nullptr, s);
+}
+
+int gpumpi_run(gpumpi_runtime* rt, const char* program, int argc, const char* const* argv,
+               const void* args, size_t args_bytes, void* stream, double* elapsed_ms) {
+    if (!rt) return GPUMPI_E_ARG;
+    const ProgramEntry* pe = find_program(program);
+    if (!pe) {
+        set_err(rt, std::string("unknown rank program: ") + (program ? program : "(null)"));
+        return GPUMPI_E_PROGRAM;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : rt->own_stream;
+    rt->err.clear();
+    if (rt->needs_reset) {
+        int rc = runtime_init_device(rt, s);
+        if (rc != GPUMPI_OK) return rc;
+    }
+    // argv strings (main.cu:24-43 copies them to managed memory)
+    std::vector<char> blob;
+    std::vector<size_t> offs;
+    for (int i = 0; i < argc; i++) {
+        offs.push_back(blob.size());
+        const char* a = argv[i];
+        blob.insert(blob.end(), a, a + std::strlen(a) + 1);
+    }
+    char** d_argv = nullptr;
+    char* d_blob = nullptr;
+    if (argc > 0) {
+        HIP_TRY(hipMalloc((void**)&d_argv, sizeof(char*) * (argc + 1)));
+        HIP_TRY(hipMalloc((void**)&d_blob, blob.size()));
+        std::vector<char*> ptrs(argc + 1, nullptr);
+        for (int i = 0; i < argc; i++) ptrs[i] = d_blob + offs[i];
+        HIP_TRY(hipMemcpy(d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_argv, ptrs.data(), sizeof(char*) * (argc + 1), hipMemcpyHostToDevice));
+    }
+    int rc = stage_args(rt, args, args_bytes, s);
+    if (rc != GPUMPI_OK) return rc;
+    hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(64), 0, s, rt->d_rt);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    rc = launch_program(rt, pe, argc, d_argv, s);
+    HIP_TRY(hipEventRecord(e1, s));
+    hipError_t se = hipStreamSynchronize(s);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (d_argv) hipFree(d_argv);
+    if (d_blob) hipFree(d_blob);
+    if (rc != GPUMPI_OK) return rc;
+    if (se != hipSuccess) {
+        set_err(rt, std::string("rank grid failed: ") + hipGetErrorString(se));
+        return GPUMPI_E_HIP;
+    }
+    if (elapsed_ms) *elapsed_ms = ms;
+    Runtime hr;
+    HIP_TRY(hipMemcpy(&hr, rt->d_rt, sizeof(Runtime), hipMemcpyDeviceToHost));
+    if (hr.abort_flag) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "device error %d on rank %d (line %d, detail %d): %s",
+                      hr.err.code, hr.err.rank, hr.err.line, hr.err.detail, hr.err.msg);
+        set_err(rt, msg);
+        rt->h.err = hr.err;
+        rt->needs_reset = true;
+        return GPUMPI_E_DEVICE;
+    }
+    rt->h.err = hr.err;
+    return hr.exit_code ? 1 : 0;
+}
+
+int gpumpi_last_device_error(gpumpi_runtime* rt, int32_t* code, int32_t* rank) {
+    if (!rt) return GPUMPI_E_ARG;
+    if (code) *code = rt->h.err.code;
+    if (rank) *rank = rt->h.err.rank;
+    return GPUMPI_OK;
+}
+
+int gpumpi_heap_stats(gpumpi_runtime* rt, int64_t* in_use, int64_t* peak, int64_t* capacity) {
+    if (!rt) return GPUMPI_E_ARG;
+    Heap hh;
+    HIP_TRY(hipMemcpy(&hh, (char*)rt->d_rt + offsetof(Runtime, heap), sizeof(Heap),
+                      hipMemcpyDeviceToHost));
+    if (in_use) *in_use = hh.bytes_in_use;
+    if (peak) *peak = hh.peak_in_use;
+    if (capacity) *capacity = (int64_t)hh.bytes;
+    return GPUMPI_OK;
+}
+
+int gpumpi_launch(const char* program, int argc, const char* const* argv, const void* args,
+                  size_t args_bytes, const gpumpi_config* cfg) {
+    gpumpi_runtime* rt = nullptr;
+    int rc = gpumpi_runtime_create(cfg, &rt);
+    if (rc != GPUMPI_OK) return rc;
+    rc = gpumpi_run(rt, program, argc, argv, args, args_bytes, nullptr, nullptr);
+    if (rc < 0) g_host_error = rt->err;
+    gpumpi_runtime_destroy(rt);
+    return rc;
+}
+
+// main.cu:49-98: "<program> <args> ---gpumpi -n N -g G -b B -s S -p P -e E"
+int gpumpi_main(int argc, const char* const* argv) {
+    int sep = -1;
+    for (int i = 0; i < argc; i++)
+        if (std::strcmp(argv[i], "---gpumpi") == 0) sep = i;
+    if (sep < 0) {
+        std::fprintf(stderr, "You should specify gpumpi related options after '---gpumpi'\n");
+        return 1;
+    }
+    if (sep < 1) {
+        std::fprintf(stderr, "GPUMPI: no rank program named before '---gpumpi'\n");
+        return 1;
+    }
+    gpumpi_config cfg;
+    gpumpi_config_default(&cfg);
+    long long n = 0, g = 0, b = 0;
+    for (int i = sep + 1; i < argc; i++) {
+        const char* o = argv[i];
+        const char* v = i + 1 < argc ? argv[i + 1] : nullptr;
+        auto num = [&](const char* s) { return s ? std::strtoll(s, nullptr, 0) : 0LL; };
+        if (!std::strcmp(o, "-h") || !std::strcmp(o, "--help")) {
+            std::printf("GPU MPI: -n numProcs -g blocksPerGrid -b threadsPerBlock (multiple of 64) "
+                        "-s stackSize -p heapSize -e pendingBufferSize -t timeoutSeconds\n");
+            return 0;
+        }
+        if (!v) break;
+        if (!std::strcmp(o, "-n")) n = num(v);
+        else if (!std::strcmp(o, "-g")) g = num(v);
+        else if (!std::strcmp(o, "-b")) b = num(v);
+        else if (!std::strcmp(o, "-s")) cfg.stack_bytes = (uint64_t)num(v);
+        else if (!std::strcmp(o, "-p")) cfg.heap_bytes = (uint64_t)num(v);
+        else if (!std::strcmp(o, "-e")) cfg.max_requests = (int32_t)num(v);
+        else if (!std::strcmp(o, "-t")) cfg.timeout_s = std::strtod(v, nullptr);
+        else { std::fprintf(stderr, "GPUMPI: unknown option %s\n", o); return 1; }
+        i++;
+    }
+    if (b % 64 != 0 || b > 256) {
+        std::fprintf(stderr, "GPUMPI: threads per block must be a multiple of 64 up to 256\n");
+        return 1;
+    }
+    int rpb = b ? (int)(b / 64) : 1;
+    if (n == 0) n = (g ? g : 1) * rpb;
+    if (g && n > g * rpb) {
+        std::fprintf(stderr, "GPUMPI: Requested number of processes %lld can't be allocated on "
+                             "%lld blocks of %d ranks\n", n, g, rpb);
+        return 1;
+    }
+    cfg.num_ranks = (int32_t)n;
+    cfg.ranks_per_block = rpb;
+    std::fprintf(stderr, "GPUMPI: Using %lld mpi processes (%d per workgroup)\n", n, rpb);
+    int rc = gpumpi_launch(argv[0], sep - 1, argv + 1, nullptr, 0, &cfg);
+    if (rc < 0) {
+        std::fprintf(stderr, "GPUMPI: %s\n", g_host_error.c_str());
+        return 1;
+    }
+    return rc;
+}
+
+}  // extern "C"
+
+// ==================================================================== batched Alltoallv
+struct gpumpi_a2av_plan {
+    A2avDev dev{};
+    gpumpi_a2av_tune tune{};
+    int grid = 0;
+    int cus = 0;
+    uint64_t last_waves = 0;
+    std::vector<void*> allocs;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    int graph_steps = 0;
+    std::string err;
+};
+
+static const void* a2av_kernel_for(int U) {
+    switch (U) {
+    case 2: return (const void*)&a2av_step_kernel<2>;
+    case 4: return (const void*)&a2av_step_kernel<4>;
+    case 8: return (const void*)&a2av_step_kernel<8>;
+    default: return (const void*)&a2av_step_kernel<16>;
+    }
+}
+
+static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
+    int U = t.lines_per_lane;
+    if (U != 2 && U != 4 && U != 8 && U != 16) return GPUMPI_E_ARG;
+    int bt = t.block_threads;
+    if (bt != 256 && bt != 512 && bt != 1024) return GPUMPI_E_ARG;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a2av_kernel_for(U), bt, 0) != hipSuccess ||
+        per_cu <= 0)
+        return GPUMPI_E_RESIDENCY;
+    int bpc = t.blocks_per_cu <= 0 ? 1 : t.blocks_per_cu;
+    if (bpc > per_cu) bpc = per_cu;  // the grid barrier needs every workgroup resident
+    pl->tune = t;
+    pl->tune.blocks_per_cu = bpc;
+    pl->grid = bpc * pl->cus;
+    pl->dev.nontemporal = t.nontemporal > 0 ? 1 : 0;
+    return GPUMPI_OK;
+}
+
+static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s) {
+    const uint64_t waves = (uint64_t)pl->grid * (pl->tune.block_threads / 64);
+    if (waves != pl->last_waves) {
+        // the arrival counter must be a multiple of the wave count at kernel start
+        hipLaunchKernelGGL(a2av_reset_kernel, dim3(1), dim3(64), 0, s, pl->dev.arrive,
+                           pl->dev.release);
+        pl->last_waves = waves;
+    }
+    void* kargs[] = {(void*)&pl->dev};
+    if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane), dim3(pl->grid),
+                        dim3(pl->tune.block_threads), kargs, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    return GPUMPI_OK;
+}
+
+static void a2av_drop_graph(gpumpi_a2av_plan* pl) {
+    if (pl->graph) hipGraphExecDestroy(pl->graph);
+    if (pl->graph_def) hipGraphDestroy(pl->graph_def);
+    pl->graph = nullptr;
+    pl->graph_def = nullptr;
+    pl->graph_steps = 0;
+}
+
+extern "C" {
+
+int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,
+                            const int32_t* send_counts, const int32_t* send_displs,
+                            void* const* recvbufs, int32_t* recv_counts, int32_t* recv_displs,
+                            const gpumpi_a2av_tune* tune, gpumpi_a2av_plan** out) {
+    if (!out || P <= 0 || elem_bytes <= 0 || !sendbufs || !send_counts || !send_displs ||
+        !recvbufs || !recv_counts || !recv_displs)
+        return GPUMPI_E_ARG;
+    gpumpi_a2av_plan* pl = new gpumpi_a2av_plan();
+    int32_t cus = 0, khz = 0;
+    if (gpumpi_device_info(&cus, &khz, nullptr) != GPUMPI_OK) {
+        delete pl;
+        return GPUMPI_E_HIP;
+    }
+    pl->cus = cus;
+    A2avDev& d = pl->dev;
+    d.P = P;
+    d.elem = elem_bytes;
+    d.sendbufs = (char* const*)sendbufs;
+    d.scounts = send_counts;
+    d.sdispls = send_displs;
+    d.recvbufs = (char* const*)recvbufs;
+    d.rcounts = recv_counts;
+    d.rdispls = recv_displs;
+    d.timeout_ticks = (uint64_t)(20.0 * (double)khz * 1000.0);
+    const uint64_t PP = (uint64_t)P * (uint64_t)P;
+    size_t ws = sizeof(A2avSeg) * PP + sizeof(uint64_t) * P + sizeof(uint64_t) * (P + 1) +
+                sizeof(int64_t) * P + 64 * 4;
+    char* w = nullptr;
+    if (hipMalloc((void**)&w, ws) != hipSuccess) {
+        delete pl;
+        return GPUMPI_E_NOMEM;
+    }
+    pl->allocs.push_back(w);
+    hipMemset(w, 0, ws);
+    d.table = (A2avSeg*)w;
+    w += sizeof(A2avSeg) * PP;
+    d.row_lines = (uint64_t*)w;
+    w += sizeof(uint64_t) * P;
+    d.row_off = (uint64_t*)w;
+    w += sizeof(uint64_t) * (P + 1);
+    d.row_bytes = (int64_t*)w;
+    w += sizeof(int64_t) * P;
+    d.total_bytes = (int64_t*)w;
+    d.arrive = (uint64_t*)(w + 64);
+    d.release = (uint64_t*)(w + 128);
+    d.error = (int32_t*)(w + 192);
+    gpumpi_a2av_tune t{512, 2, 8, 1};
+    if (tune) {
+        if (tune->block_threads > 0) t.block_threads = tune->block_threads;
+        if (tune->blocks_per_cu > 0) t.blocks_per_cu = tune->blocks_per_cu;
+        if (tune->lines_per_lane > 0) t.lines_per_lane = tune->lines_per_lane;
+        if (tune->nontemporal >= 0) t.nontemporal = tune->nontemporal;
+    }
+    int rc = a2av_configure(pl, t);
+    if (rc != GPUMPI_OK) {
+        gpumpi_a2av_plan_destroy(pl);
+        return rc;
+    }
+    hipDeviceSynchronize();
+    *out = pl;
+    return GPUMPI_OK;
+}
+
+void gpumpi_a2av_plan_destroy(gpumpi_a2av_plan* pl) {
+    if (!pl) return;
+    hipDeviceSynchronize();
+    a2av_drop_graph(pl);
+    for (void* p : pl->allocs) hipFree(p);
+    delete pl;
+}
+
+int gpumpi_a2av_execute(gpumpi_a2av_plan* pl, void* stream) {
+    if (!pl) return GPUMPI_E_ARG;
+    return a2av_launch(pl, (hipStream_t)stream);
+}
+
+int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) {
+    if (!pl || steps <= 0) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (!pl->graph || pl->graph_steps != steps) {
+        a2av_drop_graph(pl);
+        const uint64_t waves = (uint64_t)pl->grid * (pl->tune.block_threads / 64);
+        if (waves != pl->last_waves) {
+            // reset outside the capture: no memset/reset node inside the graph
+            hipLaunchKernelGGL(a2av_reset_kernel, dim3(1), dim3(64), 0, s, pl->dev.arrive,
+                               pl->dev.release);
+            pl->last_waves = waves;
+        }
+        hipStream_t cs;
+        if (hipStreamCreate(&cs) != hipSuccess) return GPUMPI_E_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return GPUMPI_E_HIP;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            hipStreamDestroy(cs);
+            return GPUMPI_E_HIP;
+        }
+        int rc = GPUMPI_OK;
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
+            void* kargs[] = {(void*)&pl->dev};
+            if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane), dim3(pl->grid),
+                                dim3(pl->tune.block_threads), kargs, 0, cs) != hipSuccess)
+                rc = GPUMPI_E_HIP;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(cs, &g);
+        hipStreamDestroy(cs);
+        if (rc != GPUMPI_OK || ee != hipSuccess) return GPUMPI_E_HIP;
+        pl->graph_def = g;
+        if (hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+            a2av_drop_graph(pl);
+            return GPUMPI_E_HIP;
+        }
+        pl->graph_steps = steps;
+    }
+    return hipGraphLaunch(pl->graph, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_a2av_get_tune(gpumpi_a2av_plan* pl, gpumpi_a2av_tune* t, int32_t* grid) {
+    if (!pl) return GPUMPI_E_ARG;
+    if (t) *t = pl->tune;
+    if (grid) *grid = pl->grid;
+    return GPUMPI_OK;
+}
+
+int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* pl, int64_t* bytes) {
+    if (!pl || !bytes) return GPUMPI_E_ARG;
+    int32_t err = 0;
+    if (hipMemcpy(bytes, pl->dev.total_bytes, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&err, pl->dev.error, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return GPUMPI_E_HIP;
+    return err ? GPUMPI_E_DEVICE : GPUMPI_OK;
+}
+
+// Sweep launch shapes on the live buffers; every candidate runs the real step (so the
+// receive buffers hold the correct exchange afterwards).
+int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* chosen,
+                         double* best_us) {
+    if (!pl) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    a2av_drop_graph(pl);
+    const int threads[] = {256, 512, 1024};
+    const int bpcs[] = {1, 2, 4, 8};
+    const int us[] = {4, 8, 16};
+    const int nts[] = {1, 0};
+    gpumpi_a2av_tune best = pl->tune;
+    double best_t = 1e30;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (int bt : threads)
+        for (int bpc : bpcs) {
+            if ((int64_t)bt * bpc > 2048) continue;  // at most 32 waves per CU
+            for (int U : us)
+                for (int nt : nts) {
+                    gpumpi_a2av_tune t{bt, bpc, U, nt};
+                    if (a2av_configure(pl, t) != GPUMPI_OK) continue;
+                    if (pl->tune.blocks_per_cu != bpc) continue;  // occupancy-clamped duplicate
+                    a2av_launch(pl, s);  // warm-up
+                    const int reps = 3;
+                    hipEventRecord(e0, s);
+                    for (int r = 0; r < reps; r++) a2av_launch(pl, s);
+                    hipEventRecord(e1, s);
+                    hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    double t_us = 1000.0 * ms / reps;
+                    if (t_us < best_t) {
+                        best_t = t_us;
+                        best = pl->tune;
+                    }
+                }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    int rc = a2av_configure(pl, best);
+    if (chosen) *chosen = pl->tune;
+    if (best_us) *best_us = best_t;
+    return rc;
+}
+
+}  // extern "C"

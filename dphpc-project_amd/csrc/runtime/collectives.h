@@ -1,0 +1,324 @@
+// Collectives of the rank runtime (the "plain" per-rank path).
+//
+// Reference algorithms (gpu_mpi/mpi.cu): Bcast is a root loop of P-1 isends, or two grid
+// barriers around a shared buffer for COMM_WORLD (mpi.cu:86-140); Reduce receives P-1
+// messages into malloc(P*N*s) at the root and folds them in rank order 0..P-1, except
+// FP64 SUM on COMM_WORLD, which uses per-element atomicAdd (non-deterministic,
+// mpi.cu:148-235); Allreduce = Reduce + Bcast (276-282); Alltoall(v) = P-1 Isend + P-1
+// Irecv + self memcpy (333-408); Gather/Scatter/Allgather are linear root loops.
+//
+// Here every collective runs over the shared address space with all P wavefronts
+// working at once:
+//   Bcast       every rank pulls from the root's buffer;
+//   Reduce      rank k reduces element slice k over all P inputs in the reference's
+//               order (out = in_0; out = op(in_r, out), r = 1..P-1) -- bit-exact with the
+//               reference's generic path for every type, FP64 included -- and writes it
+//               to the root;
+//   Allreduce   same slice reduction, result stored into every rank's receive buffer
+//               (2*P*N*s bytes moved, one barrier);
+//   Alltoall(v) each rank pulls its P incoming blocks, starting at me+1 so sources are
+//               spread;
+//   Gather(v)   each rank pushes its block into the root's buffer; Scatter(v) and
+//               Allgather(v) pull.
+#pragma once
+
+#include "comm.h"
+#include "ops.h"
+
+namespace gpumpi {
+
+__device__ __forceinline__ bool check_comm(const CommImpl* c) {
+    if (!c) {
+        GPUMPI_FAIL(kErrArgument, 0, "invalid communicator (MPI_COMM_NULL)");
+        return false;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ Bcast
+__device__ int coll_bcast(void* buffer, uint64_t bytes, int root, CommImpl* c) {
+    const int me = comm_rank_of(c);
+    if (root < 0 || root >= c->size) {
+        GPUMPI_FAIL(kErrArgument, root, "MPI_Bcast: invalid root");
+        return MPI_ERR_OTHER;
+    }
+    if (c->size == 1 || bytes == 0) return MPI_SUCCESS;
+    Stager st;
+    uint64_t seq = coll_begin(c, me);
+    if (me == root) {
+        const void* src = st.in(buffer, bytes);
+        coll_publish(c, me, seq, src, nullptr, nullptr, nullptr, (int64_t)bytes, 0, 0, 0);
+    } else {
+        if (!coll_wait_slot(c, root, seq)) return MPI_ERR_OTHER;
+        const CollSlot* s = &c->slots[root];
+        if ((uint64_t)s->a0 != bytes) {
+            GPUMPI_FAIL(kErrTruncate, (int32_t)s->a0, "MPI_Bcast: message size mismatch");
+        }
+        wave_copy(buffer, s->sbuf, bytes < (uint64_t)s->a0 ? bytes : (uint64_t)s->a0);
+    }
+    bool ok = comm_barrier(c);
+    st.release();
+    return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
+}
+
+// ------------------------------------------------------------------ Reduce / Allreduce
+// Slice of the element range owned by comm rank k: multiples of 256 elements so every
+// rank's stores stay 16-byte aligned for all element sizes.
+__device__ __forceinline__ void reduce_slice(int64_t count, int P, int k, int64_t* lo, int64_t* hi) {
+    int64_t per = (count + P - 1) / P;
+    per = (per + 255) & ~(int64_t)255;
+    *lo = (int64_t)k * per;
+    *hi = *lo + per;
+    if (*lo > count) *lo = count;
+    if (*hi > count) *hi = count;
+}
+
+struct ReduceFunctor {
+    const CommImpl* c;
+    int64_t lo, hi;
+    int root;  // -1: all ranks receive
+    template <class T, int OP>
+    __device__ void run() {
+        const int P = c->size;
+        const CollSlot* slots = c->slots;
+        for (int64_t i = lo + lane(); i < hi; i += kWave) {
+            T acc = ((const T*)slots[0].sbuf)[i];
+            int r = 1;
+            for (; r + 3 < P; r += 4) {
+                T x1 = ((const T*)slots[r].sbuf)[i];
+                T x2 = ((const T*)slots[r + 1].sbuf)[i];
+                T x3 = ((const T*)slots[r + 2].sbuf)[i];
+                T x4 = ((const T*)slots[r + 3].sbuf)[i];
+                acc = combine<T, OP>(x1, acc);
+                acc = combine<T, OP>(x2, acc);
+                acc = combine<T, OP>(x3, acc);
+                acc = combine<T, OP>(x4, acc);
+            }
+            for (; r < P; r++) acc = combine<T, OP>(((const T*)slots[r].sbuf)[i], acc);
+            if (root >= 0) {
+                ((T*)slots[root].rbuf)[i] = acc;
+            } else {
+                for (int d = 0; d < P; d++) ((T*)slots[d].rbuf)[i] = acc;
+            }
+        }
+    }
+};
+
+__device__ int coll_reduce(const void* sendbuf, void* recvbuf, int64_t count, int dt,
+                           const OpImpl* op, int root, CommImpl* c) {
+    const int P = c->size;
+    const int me = comm_rank_of(c);
+    const int es = type_size(dt);
+    if (es <= 0 || !op) {
+        GPUMPI_FAIL(kErrArgument, dt, "reduction: invalid datatype or operator");
+        return MPI_ERR_OTHER;
+    }
+    if (root >= P) {
+        GPUMPI_FAIL(kErrArgument, root, "MPI_Reduce: invalid root");
+        return MPI_ERR_OTHER;
+    }
+    if (op->builtin >= 0 && !op_allowed(dt, op->builtin)) {
+        GPUMPI_FAIL(kErrArgument, dt * 16 + op->builtin,
+                    "combination of type and operator is not allowed in reduce operation");
+        return MPI_ERR_OTHER;
+    }
+    if (count <= 0) return MPI_SUCCESS;
+    const uint64_t bytes = (uint64_t)count * (uint64_t)es;
+    Stager st;
+    const void* in = st.in(sendbuf, bytes);
+    void* out = nullptr;
+    if (root < 0 || me == root) out = st.out(recvbuf, bytes);
+    uint64_t seq = coll_begin(c, me);
+    coll_publish(c, me, seq, in, out, nullptr, nullptr, 0, 0, 0, 0);
+    if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
+    int64_t lo, hi;
+    reduce_slice(count, P, me, &lo, &hi);
+    if (lo < hi) {
+        if (op->builtin >= 0) {
+            ReduceFunctor f{c, lo, hi, root};
+            dispatch_op(dt, op->builtin, f);
+        } else {
+            // user operator: fold on lane 0 into the first destination, then replicate
+            char* dst0 = (char*)c->slots[root >= 0 ? root : 0].rbuf + lo * es;
+            uint64_t nb = (uint64_t)(hi - lo) * es;
+            wave_copy(dst0, (const char*)c->slots[0].sbuf + lo * es, nb);
+            fence_release();
+            int len = (int)(hi - lo);
+            MPI_Datatype t = (MPI_Datatype)dt;
+            for (int r = 1; r < P; r++) {
+                if (lane() == 0)
+                    op->user_fn((void*)((const char*)c->slots[r].sbuf + lo * es), dst0, &len, &t);
+                fence_acquire();
+            }
+            if (root < 0)
+                for (int d = 1; d < P; d++)
+                    wave_copy((char*)c->slots[d].rbuf + lo * es, dst0, nb);
+        }
+    }
+    bool ok = comm_barrier(c);
+    if (out && out != recvbuf) st.back(recvbuf, out, bytes);
+    st.release();
+    return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
+}
+
+// ------------------------------------------------------------------ Alltoall(v)
+// counts/displs are in elements of the respective datatype (gpu_mpi/mpi.cu:363-408).
+__device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int* sdispls,
+                              int sdt, void* recvbuf, const int* rcounts, const int* rdispls,
+                              int rdt, CommImpl* c, int uniform_scount, int uniform_rcount) {
+    const int P = c->size;
+    const int me = comm_rank_of(c);
+    const int se = type_size(sdt), re = type_size(rdt);
+    if (se <= 0 || re <= 0) {
+        GPUMPI_FAIL(kErrArgument, sdt, "alltoall: invalid datatype");
+        return MPI_ERR_OTHER;
+    }
+    const bool uniform = scounts == nullptr;
+    Stager st;
+    // extents of the local buffers (for staging of private buffers)
+    uint64_t sext = 0, rext = 0;
+    if (uniform) {
+        sext = (uint64_t)uniform_scount * se * P;
+        rext = (uint64_t)uniform_rcount * re * P;
+    } else if (is_private(sendbuf) || is_private(recvbuf)) {
+        int64_t sm = 0, rm = 0;
+        for (int i = lane(); i < P; i += kWave) {
+            int64_t a = (int64_t)sdispls[i] + scounts[i];
+            int64_t b = (int64_t)rdispls[i] + rcounts[i];
+            sm = a > sm ? a : sm;
+            rm = b > rm ? b : rm;
+        }
+        sext = (uint64_t)wave_max(sm) * se;
+        rext = (uint64_t)wave_max(rm) * re;
+    }
+    const void* sb = st.in(sendbuf, sext);
+    void* rb = st.out(recvbuf, rext);
+    const int* sc = uniform ? nullptr : (const int*)st.in(scounts, sizeof(int) * (uint64_t)P);
+    const int* sd = uniform ? nullptr : (const int*)st.in(sdispls, sizeof(int) * (uint64_t)P);
+    uint64_t seq = coll_begin(c, me);
+    coll_publish(c, me, seq, sb, rb, sc, sd, se, uniform_scount, 0, 0);
+    if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
+    for (int k = 1; k <= P; k++) {
+        const int s = (me + k) % P;
+        const CollSlot* src = &c->slots[s];
+        uint64_t n, soff, doff;
+        if (uniform) {
+            n = (uint64_t)uniform_rcount * re;
+            soff = (uint64_t)me * (uint64_t)src->a1 * (uint64_t)src->a0;
+            doff = (uint64_t)s * n;
+        } else {
+            n = (uint64_t)rcounts[s] * re;
+            soff = (uint64_t)src->displs[me] * (uint64_t)src->a0;
+            doff = (uint64_t)rdispls[s] * re;
+        }
+        if (n) wave_copy((char*)rb + doff, (const char*)src->sbuf + soff, n);
+    }
+    bool ok = comm_barrier(c);
+    if (rb != recvbuf) st.back(recvbuf, rb, rext);
+    st.release();
+    return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
+}
+
+// ------------------------------------------------------------------ Gather(v)
+// Every rank pushes its block into the root's receive buffer.
+__device__ int coll_gatherv(const void* sendbuf, uint64_t sbytes, void* recvbuf,
+                            const int* rcounts, const int* displs, int uniform_rcount,
+                            int rdt, int root, CommImpl* c, bool all) {
+    const int P = c->size;
+    const int me = comm_rank_of(c);
+    const int re = type_size(rdt);
+    if (re <= 0 || root >= P) {
+        GPUMPI_FAIL(kErrArgument, rdt, "gather: invalid datatype or root");
+        return MPI_ERR_OTHER;
+    }
+    Stager st;
+    const bool uniform = rcounts == nullptr;
+    uint64_t rext = 0;
+    if (all || me == root) {
+        if (uniform) {
+            rext = (uint64_t)uniform_rcount * re * P;
+        } else {
+            int64_t m = 0;
+            for (int i = lane(); i < P; i += kWave) {
+                int64_t a = (int64_t)displs[i] + rcounts[i];
+                m = a > m ? a : m;
+            }
+            rext = (uint64_t)wave_max(m) * re;
+        }
+    }
+    const void* sb = st.in(sendbuf, sbytes);
+    void* rb = st.out(recvbuf, rext);
+    const int* rc = (uniform || !(all || me == root)) ? nullptr
+                        : (const int*)st.in(rcounts, sizeof(int) * (uint64_t)P);
+    const int* rd = (uniform || !(all || me == root)) ? nullptr
+                        : (const int*)st.in(displs, sizeof(int) * (uint64_t)P);
+    uint64_t seq = coll_begin(c, me);
+    coll_publish(c, me, seq, sb, rb, rc, rd, (int64_t)sbytes, uniform_rcount, 0, 0);
+    if (all) {
+        // Allgather(v): pull every rank's block into my buffer
+        if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
+        for (int k = 0; k < P; k++) {
+            const int s = (me + k) % P;
+            uint64_t doff = uniform ? (uint64_t)s * uniform_rcount * re : (uint64_t)rd[s] * re;
+            uint64_t n = uniform ? (uint64_t)uniform_rcount * re : (uint64_t)rc[s] * re;
+            uint64_t avail = (uint64_t)c->slots[s].a0;
+            wave_copy((char*)rb + doff, c->slots[s].sbuf, n < avail ? n : avail);
+        }
+    } else {
+        if (!coll_wait_slot(c, root, seq)) return MPI_ERR_OTHER;
+        const CollSlot* rs = &c->slots[root];
+        uint64_t doff = uniform ? (uint64_t)me * rs->a1 * re : (uint64_t)rs->displs[me] * re;
+        uint64_t n = uniform ? (uint64_t)rs->a1 * re : (uint64_t)rs->counts[me] * re;
+        wave_copy((char*)rs->rbuf + doff, sb, n < sbytes ? n : sbytes);
+    }
+    bool ok = comm_barrier(c);
+    if (rb != recvbuf) st.back(recvbuf, rb, rext);
+    st.release();
+    return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
+}
+
+// ------------------------------------------------------------------ Scatter(v)
+__device__ int coll_scatterv(const void* sendbuf, const int* scounts, const int* displs,
+                             int uniform_scount, int sdt, void* recvbuf, uint64_t rbytes,
+                             int root, CommImpl* c) {
+    const int P = c->size;
+    const int me = comm_rank_of(c);
+    const int se = type_size(sdt);
+    if (se <= 0 || root >= P) {
+        GPUMPI_FAIL(kErrArgument, sdt, "scatter: invalid datatype or root");
+        return MPI_ERR_OTHER;
+    }
+    Stager st;
+    const bool uniform = scounts == nullptr;
+    const void* sb = nullptr;
+    const int* sc = nullptr;
+    const int* sd = nullptr;
+    if (me == root) {
+        uint64_t sext;
+        if (uniform) {
+            sext = (uint64_t)uniform_scount * se * P;
+        } else {
+            int64_t m = 0;
+            for (int i = lane(); i < P; i += kWave) {
+                int64_t a = (int64_t)displs[i] + scounts[i];
+                m = a > m ? a : m;
+            }
+            sext = (uint64_t)wave_max(m) * se;
+            sc = (const int*)st.in(scounts, sizeof(int) * (uint64_t)P);
+            sd = (const int*)st.in(displs, sizeof(int) * (uint64_t)P);
+        }
+        sb = st.in(sendbuf, sext);
+    }
+    uint64_t seq = coll_begin(c, me);
+    coll_publish(c, me, seq, sb, nullptr, sc, sd, se, uniform_scount, 0, 0);
+    if (!coll_wait_slot(c, root, seq)) return MPI_ERR_OTHER;
+    const CollSlot* rs = &c->slots[root];
+    uint64_t soff = uniform ? (uint64_t)me * rs->a1 * rs->a0 : (uint64_t)rs->displs[me] * rs->a0;
+    uint64_t n = uniform ? (uint64_t)rs->a1 * rs->a0 : (uint64_t)rs->counts[me] * rs->a0;
+    wave_copy(recvbuf, (const char*)rs->sbuf + soff, n < rbytes ? n : rbytes);
+    bool ok = comm_barrier(c);
+    st.release();
+    return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
+}
+
+}  // namespace gpumpi

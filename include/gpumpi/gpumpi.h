@@ -1,0 +1,142 @@
+/*
+ * gpumpi host C ABI (libgpumpi.so) -- launches rank grids on one MI355X.
+ *
+ * Plain C types only (pointers, sizes, ints); device buffers are passed as raw device
+ * addresses and streams as hipStream_t cast to void*.  Every entry point returns 0 on
+ * success, a positive program exit status, or a negative gpumpi_status code.
+ *
+ * Reference interfaces replaced:
+ *   gpumpi_main ............ gpu_libs/gpu_main/main.cu:134-322 (the launcher's main():
+ *                            "<prog args> ---gpumpi -n -g -b -s -p -e", exit code of the
+ *                            rank program, main.cu:49-98, 127-130, 321)
+ *   gpumpi_config .......... the ---gpumpi options (main.cu:71-79) plus the hard-coded
+ *                            recvListSize (main.cu:219) and pendingBufferSize (-e)
+ *   gpumpi_runtime_create /
+ *   gpumpi_run ............. SharedState::Holder + the single cudaLaunchKernel of
+ *                            __gpu_main_caller (main.cu:217-289), split so one runtime
+ *                            serves many launches (the plain per-call path)
+ *   gpumpi_last_error ...... CudaMPI::recordError / printLastError (gpu_libc/assert.cu:24-71)
+ *   gpumpi_a2av_*  ......... no reference counterpart: the batched persistent-kernel
+ *                            MPI_Alltoallv step (gpu_mpi/mpi.cu:363-408 semantics)
+ *   gpumpi_serve_* ......... no reference counterpart: the device-resident job server
+ *
+ * A reference-side maintainer binds these with ctypes/cffi or links them directly; the
+ * exact bindings are listed in INTEGRATION.md.
+ */
+#ifndef GPUMPI_GPUMPI_H
+#define GPUMPI_GPUMPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gpumpi_status {
+    GPUMPI_OK = 0,
+    GPUMPI_E_ARG = -1,          /* invalid argument or configuration */
+    GPUMPI_E_HIP = -2,          /* a HIP runtime call failed */
+    GPUMPI_E_RESIDENCY = -3,    /* the rank grid cannot be co-resident on the GPU */
+    GPUMPI_E_PROGRAM = -4,      /* unknown rank program */
+    GPUMPI_E_DEVICE = -5,       /* the device runtime recorded an error (see last_error) */
+    GPUMPI_E_NOMEM = -6         /* device memory allocation failed */
+};
+
+typedef struct gpumpi_config {
+    int32_t num_ranks;          /* -n: MPI ranks (one wavefront each) */
+    int32_t ranks_per_block;    /* wavefronts (ranks) per workgroup, 1..4 (-b / 64); 0 = 1 */
+    int32_t ring_slots;         /* envelope ring slots per rank; 0 = 256 */
+    int32_t max_requests;       /* -e: request pool per rank; 0 = 1024 */
+    int32_t max_unexpected;     /* unexpected-message list per rank; 0 = 1024 */
+    int32_t reserved0;
+    uint64_t heap_bytes;        /* -p: device heap arena; 0 = 1 GiB */
+    uint64_t stack_bytes;       /* -s: per-lane stack limit; 0 = runtime default */
+    double timeout_s;           /* bound on every device-side wait; 0 = 20 s */
+} gpumpi_config;
+
+typedef struct gpumpi_runtime gpumpi_runtime;
+
+void gpumpi_config_default(gpumpi_config* cfg);
+
+/* Create / destroy a rank-grid runtime (mailboxes, request pools, heap, COMM_WORLD). */
+int gpumpi_runtime_create(const gpumpi_config* cfg, gpumpi_runtime** out);
+void gpumpi_runtime_destroy(gpumpi_runtime* rt);
+
+/* Launch the built-in rank program `program` on every rank and wait for it.
+ * args/args_bytes is copied to device memory and visible to ranks as the program's
+ * argument block.  Returns 0, 1 if any rank returned non-zero (main.cu:127-130), or a
+ * negative gpumpi_status.  *elapsed_ms (optional) receives the kernel's wall time. */
+int gpumpi_run(gpumpi_runtime* rt, const char* program, int argc, const char* const* argv,
+               const void* args, size_t args_bytes, void* stream, double* elapsed_ms);
+
+/* Launch without waiting (the caller synchronises the stream). */
+int gpumpi_run_async(gpumpi_runtime* rt, const char* program, const void* args,
+                     size_t args_bytes, void* stream);
+
+/* One-shot: create, run, destroy (what main.cu does for one converted program). */
+int gpumpi_launch(const char* program, int argc, const char* const* argv, const void* args,
+                  size_t args_bytes, const gpumpi_config* cfg);
+
+/* CLI form: argv = "<program> <program args> ---gpumpi -n N [-g G] [-b B] [-s S] [-p P]
+ * [-e E] [-t seconds]".  Returns the program's exit code as main.cu:321 does. */
+int gpumpi_main(int argc, const char* const* argv);
+
+/* Text of the last error (device error record or host failure); "" when none. */
+const char* gpumpi_last_error(gpumpi_runtime* rt);
+/* Device error record of the last run: code, rank (negative values when none). */
+int gpumpi_last_device_error(gpumpi_runtime* rt, int32_t* code, int32_t* rank);
+
+/* Number of built-in rank programs and their names. */
+int gpumpi_program_count(void);
+const char* gpumpi_program_name(int i);
+
+/* Heap statistics of a runtime: bytes of live blocks, peak, capacity. */
+int gpumpi_heap_stats(gpumpi_runtime* rt, int64_t* in_use, int64_t* peak, int64_t* capacity);
+
+/* Device properties used by the launcher (CU count, wall-clock rate in kHz). */
+int gpumpi_device_info(int32_t* num_cus, int32_t* wallclock_khz, int32_t* max_waves_per_cu);
+
+/* ------------------------------------------------------------------------------------
+ * Batched Alltoallv step (persistent kernel, hipGraph replay, autotuned launch shape).
+ *
+ * Describes one MPI_Alltoallv over P ranks whose buffers are already resident in HBM:
+ *   sendbufs[P], recvbufs[P]        device arrays of device pointers
+ *   send_counts[P*P]                device int32, [s][d] = elements rank s sends to d
+ *   send_displs[P*P]                device int32, [s][d] = element displacement in s's sendbuf
+ * The step performs the Alltoall of the counts (recv_counts[d][s] = send_counts[s][d]),
+ * the receive displacements (exclusive prefix over s, written to recv_displs[d][s]) and
+ * the payload exchange, all in ONE kernel launch.  recv_counts/recv_displs are device
+ * int32 [P*P] outputs.  elem_bytes is the datatype size.
+ * ------------------------------------------------------------------------------------ */
+typedef struct gpumpi_a2av_plan gpumpi_a2av_plan;
+
+typedef struct gpumpi_a2av_tune {
+    int32_t block_threads;      /* 256 / 512 / 1024; 0 = autotune */
+    int32_t blocks_per_cu;      /* persistent blocks per CU; 0 = autotune */
+    int32_t lines_per_lane;     /* 16-byte vectors in flight per lane: 2,4,8; 0 = autotune */
+    int32_t nontemporal;        /* 1 = non-temporal stores, 0 = cached, -1 = autotune */
+} gpumpi_a2av_tune;
+
+int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,
+                            const int32_t* send_counts, const int32_t* send_displs,
+                            void* const* recvbufs, int32_t* recv_counts,
+                            int32_t* recv_displs, const gpumpi_a2av_tune* tune,
+                            gpumpi_a2av_plan** out);
+void gpumpi_a2av_plan_destroy(gpumpi_a2av_plan* plan);
+/* Run one step on `stream` (asynchronous). */
+int gpumpi_a2av_execute(gpumpi_a2av_plan* plan, void* stream);
+/* Capture `steps_per_graph` steps into a hipGraph once, then replay it. */
+int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* plan, int32_t steps_per_graph, void* stream);
+/* Sweep launch shapes on the live buffers, keep the fastest; returns the chosen shape. */
+int gpumpi_a2av_autotune(gpumpi_a2av_plan* plan, void* stream, gpumpi_a2av_tune* chosen,
+                         double* best_us);
+/* Launch shape in use and the number of workgroups of the persistent grid. */
+int gpumpi_a2av_get_tune(gpumpi_a2av_plan* plan, gpumpi_a2av_tune* tune, int32_t* grid);
+/* Total payload bytes moved by the last completed step (reads them back from the device). */
+int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* plan, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

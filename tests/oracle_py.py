@@ -1,0 +1,97 @@
+"""Python-side helpers around the oracle's C restatement (test infrastructure only)."""
+import ctypes
+
+import numpy as np
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class NpbClass(ctypes.Structure):
+    _fields_ = [("total_keys_log2", ctypes.c_int), ("max_key_log2", ctypes.c_int),
+                ("num_buckets_log2", ctypes.c_int), ("cls", ctypes.c_char)]
+
+
+def npb_class(orc, cls):
+    c = NpbClass()
+    assert orc.orc_npb_class_params(ctypes.c_char(cls.encode()), ctypes.byref(c)) == 0
+    return c
+
+
+def alltoallv(orc, sendbufs, counts, sdispls, elem, recv_totals, rdispls=None):
+    """Oracle Alltoallv: returns the list of receive buffers (numpy uint8 views)."""
+    P = len(sendbufs)
+    rcounts = np.ascontiguousarray(counts.T).astype(np.int32)
+    if rdispls is None:
+        rdispls = np.zeros_like(rcounts)
+        rdispls[:, 1:] = np.cumsum(rcounts, axis=1)[:, :-1]
+    recv = [np.zeros(max(1, int(n)) * elem, dtype=np.uint8) for n in recv_totals]
+    sb = (ctypes.c_void_p * P)(*[b.ctypes.data for b in sendbufs])
+    rb = (ctypes.c_void_p * P)(*[b.ctypes.data for b in recv])
+    counts = np.ascontiguousarray(counts, dtype=np.int32)
+    sdispls = np.ascontiguousarray(sdispls, dtype=np.int32)
+    orc.orc_alltoallv(P, elem, elem, sb, ptr(counts), ptr(sdispls), rb, ptr(rcounts), ptr(rdispls))
+    return recv, rcounts, rdispls
+
+
+def reduce(orc, datatype, op, inputs):
+    """Oracle reduction in the reference's fold order over a list of numpy arrays."""
+    P = len(inputs)
+    out = np.zeros_like(inputs[0])
+    ins = (ctypes.c_void_p * P)(*[x.ctypes.data for x in inputs])
+    count = inputs[0].size if inputs[0].dtype.kind != "c" else inputs[0].size
+    rc = orc.orc_reduce(P, datatype, op, ins, ctypes.c_int64(count), ptr(out))
+    return rc, out
+
+
+def fill_int(orc, seed, rank, n, it=0):
+    out = np.zeros(n, dtype=np.int32)
+    orc.orc_fill_int(ctypes.c_uint64(seed), rank, ctypes.c_uint32(it), ptr(out), ctypes.c_int64(n))
+    return out
+
+
+def fill_normal(orc, seed, rank, n, it=0):
+    out = np.zeros(n, dtype=np.float64)
+    orc.orc_fill_normal(ctypes.c_uint64(seed), rank, ctypes.c_uint32(it), ptr(out), ctypes.c_int64(n))
+    return out
+
+
+def fill_bytes(orc, seed, rank, n, it=0):
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    orc.orc_fill_bytes(ctypes.c_uint64(seed), rank, ctypes.c_uint32(it), ptr(out), ctypes.c_int64(n))
+    return out[:n]
+
+
+def keys_uniform(orc, seed, rank, n, key_bits=23, step=0):
+    out = np.zeros(n, dtype=np.uint32)
+    orc.orc_fill_keys_uniform(ctypes.c_uint64(seed), rank, ctypes.c_uint32(step), ptr(out),
+                              ctypes.c_int64(n), key_bits)
+    return out
+
+
+def keys_zipf(orc, seed, rank, n, cdf, step=0):
+    out = np.zeros(n, dtype=np.uint32)
+    orc.orc_fill_keys_zipf(ctypes.c_uint64(seed), rank, ctypes.c_uint32(step), ptr(cdf),
+                           ctypes.c_int64(cdf.size), ptr(out), ctypes.c_int64(n))
+    return out
+
+
+def zipf_cdf(nkeys, s=1.1):
+    """CDF table of Zipf(s) over key ranks 1..nkeys (key = rank-1), sequential sums."""
+    w = np.arange(1, nkeys + 1, dtype=np.float64) ** (-s)
+    c = np.cumsum(w)
+    return c / c[-1]
+
+
+def bucket_plan(orc, keys, key_bits, bucket_bits):
+    P, n = keys.shape
+    kb1 = np.zeros_like(keys)
+    sc = np.zeros((P, P), dtype=np.int32)
+    sd = np.zeros((P, P), dtype=np.int32)
+    lo = np.zeros(P, dtype=np.int32)
+    hi = np.zeros(P, dtype=np.int32)
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    orc.orc_bucket_plan(P, ctypes.c_int64(n), key_bits, bucket_bits, ptr(keys), ptr(kb1), ptr(sc),
+                        ptr(sd), ptr(lo), ptr(hi))
+    return kb1, sc, sd, lo, hi

@@ -1,0 +1,44 @@
+"""The C-ABI library loads and exports every symbol include/gpumpi/gpumpi.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "gpumpi", "gpumpi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpumpi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_declared_symbols(pkg):
+    lib = pkg.load_library()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"missing exports: {missing}"
+    assert set(names) == set(pkg.exported_symbols())
+
+
+def test_program_registry(pkg):
+    names = pkg.program_names()
+    for p in pkg.programs.REFERENCE_TESTS:
+        assert p in names
+    for p in ["mcpi", "p2p", "coll", "churn", "exchange"]:
+        assert p in names
+
+
+def test_config_defaults(pkg):
+    c = pkg.default_config(num_ranks=4)
+    assert c.num_ranks == 4 and c.ring_slots == 256 and c.max_requests == 1024
+    assert c.heap_bytes == 1 << 30 and abs(c.timeout_s - 20.0) < 1e-9
+
+
+def test_mpi_header_datatype_order(pkg):
+    """MPI_Datatype keeps the reference's enum order (mpi_types_list.cuh:4-33)."""
+    src = open(os.path.join(ROOT, "include", "gpumpi", "mpi.h")).read()
+    body = re.search(r"enum MPI_Datatype \{(.*?)\};", src, re.S).group(1)
+    names = [x.strip() for x in body.replace("\n", " ").split(",") if x.strip()]
+    assert names == pkg.DATATYPES
