@@ -81,8 +81,12 @@ _SIGNATURES = {
     "gpumpi_a2av_execute": (ctypes.c_int, [_P, _P]),
     "gpumpi_a2av_graph_replay": (ctypes.c_int, [_P, _I32, _P]),
     "gpumpi_a2av_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(A2avTune), ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_a2av_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_get_tune": (ctypes.c_int, [_P, ctypes.POINTER(A2avTune), ctypes.POINTER(_I32)]),
     "gpumpi_a2av_payload_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "gpumpi_gen_keys": (ctypes.c_int, [_P, _I32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
+                                       _I32, _I32, _P, ctypes.c_int64, _P]),
+    "gpumpi_bucket_plan": (ctypes.c_int, [_P, _I32, ctypes.c_int64, _I32, _I32, _P, _P, _P, _P, _P, _P]),
 }
 
 
@@ -109,6 +113,25 @@ def load_library():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def gen_keys(keys_ptr, P, n, seed, step=0, dist="uniform", key_bits=23, cdf_ptr=0, ncdf=0,
+             stream=0):
+    """Fill keys[P][n] (device uint32) with the workload's seeded keys."""
+    d = {"uniform": 0, "zipf": 1, "npb": 2}[dist]
+    rc = load_library().gpumpi_gen_keys(keys_ptr, P, n, seed, step, d, key_bits,
+                                        cdf_ptr or None, ncdf, stream or None)
+    if rc != OK:
+        raise GpumpiError(f"gpumpi_gen_keys failed ({rc})")
+
+
+def bucket_plan(keys_ptr, P, n, key_bits, bucket_bits, kb1_ptr, sc_ptr, sd_ptr, lo_ptr, hi_ptr,
+                stream=0):
+    """NPB-style bucket plan on device buffers (see gpumpi_bucket_plan)."""
+    rc = load_library().gpumpi_bucket_plan(keys_ptr, P, n, key_bits, bucket_bits, kb1_ptr,
+                                           sc_ptr, sd_ptr, lo_ptr, hi_ptr, stream or None)
+    if rc != OK:
+        raise GpumpiError(f"gpumpi_bucket_plan failed ({rc})")
 
 
 def exported_symbols():
@@ -252,6 +275,18 @@ class A2avPlan:
         d = t.as_dict()
         d["grid"] = g.value
         return d
+
+    def phase_times(self, enable=None):
+        """enable=True arms/resets the kernel's phase timestamps, False disarms; with no
+        argument returns [phase1_us, barrier_us, total_us] of the last armed launch."""
+        if enable is not None:
+            self.lib.gpumpi_a2av_phase_times(self.h, 1 if enable else 0, None)
+            return None
+        out = (ctypes.c_double * 3)()
+        rc = self.lib.gpumpi_a2av_phase_times(self.h, -1, out)
+        if rc != OK:
+            raise GpumpiError("phase times not armed")
+        return list(out)
 
     def payload_bytes(self):
         b = ctypes.c_int64()

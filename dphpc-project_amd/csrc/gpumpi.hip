@@ -3,6 +3,7 @@
 // is header-only and is inlined into every rank program).
 #include "runtime/mpi_api.h"
 #include "runtime/batched.h"
+#include "runtime/bucket.h"
 #include "programs/reference_tests.h"
 #include "programs/workloads.h"
 #include "programs/bucket_sort.h"
@@ -590,12 +591,12 @@ struct gpumpi_a2av_plan {
     std::string err;
 };
 
-static const void* a2av_kernel_for(int U) {
+static const void* a2av_kernel_for(int U, bool nt) {
     switch (U) {
-    case 2: return (const void*)&a2av_step_kernel<2>;
-    case 4: return (const void*)&a2av_step_kernel<4>;
-    case 8: return (const void*)&a2av_step_kernel<8>;
-    default: return (const void*)&a2av_step_kernel<16>;
+    case 2: return nt ? (const void*)&a2av_step_kernel<2, true> : (const void*)&a2av_step_kernel<2, false>;
+    case 4: return nt ? (const void*)&a2av_step_kernel<4, true> : (const void*)&a2av_step_kernel<4, false>;
+    case 8: return nt ? (const void*)&a2av_step_kernel<8, true> : (const void*)&a2av_step_kernel<8, false>;
+    default: return nt ? (const void*)&a2av_step_kernel<16, true> : (const void*)&a2av_step_kernel<16, false>;
     }
 }
 
@@ -605,7 +606,7 @@ static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
     int bt = t.block_threads;
     if (bt != 256 && bt != 512 && bt != 1024) return GPUMPI_E_ARG;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a2av_kernel_for(U), bt, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a2av_kernel_for(U, t.nontemporal > 0), bt, 0) != hipSuccess ||
         per_cu <= 0)
         return GPUMPI_E_RESIDENCY;
     int bpc = t.blocks_per_cu <= 0 ? 1 : t.blocks_per_cu;
@@ -618,15 +619,15 @@ static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
 }
 
 static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s) {
-    const uint64_t waves = (uint64_t)pl->grid * (pl->tune.block_threads / 64);
+    const uint64_t waves = (uint64_t)pl->grid;
     if (waves != pl->last_waves) {
-        // the arrival counter must be a multiple of the wave count at kernel start
+        // the arrival counter must be a multiple of the workgroup count at kernel start
         hipLaunchKernelGGL(a2av_reset_kernel, dim3(1), dim3(64), 0, s, pl->dev.arrive,
                            pl->dev.release);
         pl->last_waves = waves;
     }
     void* kargs[] = {(void*)&pl->dev};
-    if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane), dim3(pl->grid),
+    if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0), dim3(pl->grid),
                         dim3(pl->tune.block_threads), kargs, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
     return GPUMPI_OK;
@@ -688,6 +689,7 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
     d.arrive = (uint64_t*)(w + 64);
     d.release = (uint64_t*)(w + 128);
     d.error = (int32_t*)(w + 192);
+    d.stamps = nullptr;
     gpumpi_a2av_tune t{512, 2, 8, 1};
     if (tune) {
         if (tune->block_threads > 0) t.block_threads = tune->block_threads;
@@ -723,7 +725,7 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) 
     hipStream_t s = (hipStream_t)stream;
     if (!pl->graph || pl->graph_steps != steps) {
         a2av_drop_graph(pl);
-        const uint64_t waves = (uint64_t)pl->grid * (pl->tune.block_threads / 64);
+        const uint64_t waves = (uint64_t)pl->grid;
         if (waves != pl->last_waves) {
             // reset outside the capture: no memset/reset node inside the graph
             hipLaunchKernelGGL(a2av_reset_kernel, dim3(1), dim3(64), 0, s, pl->dev.arrive,
@@ -740,7 +742,7 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) 
         int rc = GPUMPI_OK;
         for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
             void* kargs[] = {(void*)&pl->dev};
-            if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane), dim3(pl->grid),
+            if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0), dim3(pl->grid),
                                 dim3(pl->tune.block_threads), kargs, 0, cs) != hipSuccess)
                 rc = GPUMPI_E_HIP;
         }
@@ -756,6 +758,39 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) 
         pl->graph_steps = steps;
     }
     return hipGraphLaunch(pl->graph, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_a2av_phase_times(gpumpi_a2av_plan* pl, int32_t enable, double* us) {
+    if (!pl) return GPUMPI_E_ARG;
+    if (enable > 0) {
+        if (!pl->dev.stamps) {
+            void* q = nullptr;
+            if (hipMalloc(&q, 64) != hipSuccess) return GPUMPI_E_NOMEM;
+            pl->allocs.push_back(q);
+            pl->dev.stamps = (uint64_t*)q;
+            a2av_drop_graph(pl);
+        }
+        uint64_t init[4] = {~0ull, 0, 0, 0};
+        if (hipMemcpy(pl->dev.stamps, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
+            return GPUMPI_E_HIP;
+        return GPUMPI_OK;
+    }
+    if (enable == 0) {
+        pl->dev.stamps = nullptr;
+        a2av_drop_graph(pl);
+        return GPUMPI_OK;
+    }
+    if (!pl->dev.stamps || !us) return GPUMPI_E_ARG;
+    uint64_t st[4];
+    if (hipMemcpy(st, pl->dev.stamps, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+        return GPUMPI_E_HIP;
+    int32_t khz = 0;
+    gpumpi_device_info(nullptr, &khz, nullptr);
+    const double tick_us = 1000.0 / (double)khz;
+    us[0] = (double)(st[1] - st[0]) * tick_us;  // phase 1 (slowest wave)
+    us[1] = (double)(st[2] - st[0]) * tick_us;  // through the grid barrier
+    us[2] = (double)(st[3] - st[0]) * tick_us;  // whole step
+    return GPUMPI_OK;
 }
 
 int gpumpi_a2av_get_tune(gpumpi_a2av_plan* pl, gpumpi_a2av_tune* t, int32_t* grid) {
@@ -818,6 +853,66 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
     int rc = a2av_configure(pl, best);
     if (chosen) *chosen = pl->tune;
     if (best_us) *best_us = best_t;
+    return rc;
+}
+
+}  // extern "C"
+
+// ==================================================================== bucket-sort caller
+extern "C" {
+
+int gpumpi_gen_keys(uint32_t* keys, int32_t P, int64_t n, uint64_t seed, uint32_t step,
+                    int32_t dist, int32_t key_bits, const double* cdf, int64_t ncdf, void* stream) {
+    if (!keys || P <= 0 || n < 0 || key_bits < 1 || key_bits > 31 || dist < 0 || dist > 2 ||
+        (dist == 1 && (!cdf || ncdf <= 0)))
+        return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    int64_t total = (int64_t)P * n;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(gen_keys_kernel, dim3(blocks), dim3(256), 0, s, keys, P, n, seed, step,
+                       dist, key_bits, cdf, ncdf);
+    return hipGetLastError() == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_bucket_plan(const uint32_t* keys, int32_t P, int64_t n, int32_t key_bits,
+                       int32_t bucket_bits, uint32_t* key_buff1, int32_t* send_counts,
+                       int32_t* send_displs, int32_t* bucket_lo, int32_t* bucket_hi,
+                       void* stream) {
+    if (!keys || P <= 0 || n <= 0 || bucket_bits < 1 || bucket_bits > 12 ||
+        bucket_bits > key_bits || !key_buff1 || !send_counts || !send_displs || !bucket_lo ||
+        !bucket_hi)
+        return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = 1 << bucket_bits;
+    const int shift = key_bits - bucket_bits;
+    uint32_t* hist = nullptr;
+    int64_t* totals = nullptr;
+    if (hipMalloc((void**)&hist, sizeof(uint32_t) * (size_t)P * nb) != hipSuccess) return GPUMPI_E_NOMEM;
+    if (hipMalloc((void**)&totals, sizeof(int64_t) * nb) != hipSuccess) {
+        hipFree(hist);
+        return GPUMPI_E_NOMEM;
+    }
+    int rc = GPUMPI_OK;
+    if (hipMemsetAsync(hist, 0, sizeof(uint32_t) * (size_t)P * nb, s) != hipSuccess) rc = GPUMPI_E_HIP;
+    int chunks = (int)((n + 256 * 64 - 1) / (256 * 64));
+    if (chunks > 64) chunks = 64;
+    hipLaunchKernelGGL(bucket_hist_kernel, dim3(chunks, P), dim3(256), sizeof(uint32_t) * nb, s,
+                       keys, n, shift, nb, hist);
+    hipLaunchKernelGGL(bucket_totals_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, hist, P, nb,
+                       totals);
+    hipLaunchKernelGGL(bucket_splitters_kernel, dim3(1), dim3(64), 0, s, totals, P, nb, n,
+                       bucket_lo, bucket_hi);
+    hipLaunchKernelGGL(bucket_counts_kernel, dim3((P + 3) / 4), dim3(256), 0, s, hist, P, nb,
+                       bucket_lo, bucket_hi, send_counts, send_displs);
+    hipLaunchKernelGGL(bucket_scatter_kernel, dim3((P + 3) / 4), dim3(256),
+                       sizeof(uint32_t) * 4 * nb, s, keys, P, n, shift, nb, bucket_bits, hist,
+                       key_buff1);
+    if (hipGetLastError() != hipSuccess) rc = GPUMPI_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) rc = GPUMPI_E_HIP;
+    hipFree(hist);
+    hipFree(totals);
     return rc;
 }
 

@@ -1,0 +1,198 @@
+// Device kernels of the bucket-sort caller of the exchange (the NPB IS rank() shape,
+// toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI/IS/is.c:631-928) for the batched path:
+//   gen_keys      seeded keys: uniform [0, 2^key_bits), Zipf(s) by inverse CDF, or the
+//                 NPB IS LCG (randlc/find_my_seed/create_seq, is.c:406-537) evaluated in
+//                 exact integer arithmetic with per-thread jump-ahead
+//   bucket_hist   per-rank bucket histogram (LDS-privatised)
+//   bucket_split  NPB splitters (is.c:721-751): rank j takes buckets until the global
+//                 running total reaches (j+1)*n; per-(source, destination) send counts
+//                 and displacements
+//   bucket_scatter stable bucketing (is.c:680-688) -- one wavefront per rank, 64 keys per
+//                 step, each lane's rank among equal buckets from `bits` ballots
+//   local_rank    counting sort of each rank's received keys over its bucket range
+//                 (is.c:785-833, placement as full_verify is.c:545-560)
+#pragma once
+
+#include "philox.h"
+#include "common.h"
+
+namespace gpumpi {
+
+constexpr uint64_t kNpbA = 1220703125ull;
+constexpr uint64_t kNpbSeed = 314159265ull;
+constexpr uint64_t kMask46 = (1ull << 46) - 1;
+
+__host__ __device__ __forceinline__ uint64_t lcg_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = (r * a) & kMask46;
+        a = (a * a) & kMask46;
+        e >>= 1;
+    }
+    return r;
+}
+
+// find_my_seed (is.c:470-510): s * (a^(2^mq))^kn mod 2^46 with mq = floor(log2(nn/np)).
+__host__ __device__ __forceinline__ uint64_t npb_rank_seed(int kn, int np, int64_t nn) {
+    int64_t nq = nn / np;
+    int mq = 0;
+    for (; nq > 1; mq++, nq /= 2) {}
+    uint64_t an = kNpbA;
+    for (int i = 0; i < mq; i++) an = (an * an) & kMask46;
+    return (kNpbSeed * lcg_pow(an, (uint64_t)kn)) & kMask46;
+}
+
+__global__ void gen_keys_kernel(uint32_t* keys, int P, int64_t n, uint64_t seed, uint32_t step,
+                                int dist, int key_bits, const double* cdf, int64_t ncdf) {
+    const int64_t total = (int64_t)P * n;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(g / n);
+        const int64_t i = g - (int64_t)r * n;
+        uint32_t key;
+        if (dist == 0) {
+            u32x4v v = rank_stream(seed, r, step, (uint64_t)(i >> 2), kStreamKeys);
+            key = pick(v, (int)(i & 3)) >> (32 - key_bits);
+        } else if (dist == 1) {
+            u32x4v v = rank_stream(seed, r, step, (uint64_t)(i >> 1), kStreamZipf);
+            const int h = 2 * (int)(i & 1);
+            const double u = unit53(pick(v, h), pick(v, h + 1));
+            int64_t lo = 0, hi = ncdf - 1;
+            while (lo < hi) {
+                int64_t mid = lo + (hi - lo) / 2;
+                if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+            }
+            key = (uint32_t)lo;
+        } else {
+            // NPB IS create_seq: key = (x1+x2+x3+x4) >> (48 - max_key_log2), where the x's
+            // are the next four LCG states of this rank's sequence (starts at the rank seed).
+            uint64_t x = npb_rank_seed(r, P, 4 * total);
+            x = (x * lcg_pow(kNpbA, 4ull * (uint64_t)i)) & kMask46;
+            uint64_t s = 0;
+            for (int j = 0; j < 4; j++) {
+                x = (x * kNpbA) & kMask46;
+                s += x;
+            }
+            key = (uint32_t)(s >> (48 - key_bits));
+        }
+        keys[g] = key;
+    }
+}
+
+// grid: (chunks per rank, P); hist: [P][nb]
+__global__ void bucket_hist_kernel(const uint32_t* keys, int64_t n, int shift, int nb,
+                                   uint32_t* hist) {
+    extern __shared__ uint32_t lh[];
+    const int r = blockIdx.y;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) lh[b] = 0;
+    __syncthreads();
+    const uint32_t* k = keys + (int64_t)r * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&lh[k[i] >> shift], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        if (lh[b]) atomicAdd(&hist[(int64_t)r * nb + b], lh[b]);
+}
+
+// totals[b] = sum over ranks of hist[r][b]
+__global__ void bucket_totals_kernel(const uint32_t* hist, int P, int nb, int64_t* totals) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        int64_t s = 0;
+        for (int r = 0; r < P; r++) s += hist[(int64_t)r * nb + b];
+        totals[b] = s;
+    }
+}
+
+// NPB splitters (is.c:721-751), one thread: bucket range [lo[j], hi[j]] of rank j.
+__global__ void bucket_splitters_kernel(const int64_t* totals, int P, int nb, int64_t n,
+                                        int32_t* lo, int32_t* hi) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t acc = 0;
+    int j = 0;
+    int prev = -1;
+    for (int i = 0; i < nb; i++) {
+        acc += totals[i];
+        if (acc >= (int64_t)(j + 1) * n && j < P) {
+            lo[j] = prev + 1;
+            hi[j] = i;
+            prev = i;
+            j++;
+        }
+    }
+    for (; j < P; j++) {
+        lo[j] = 1;
+        hi[j] = 0;
+    }
+}
+
+// One wavefront per source rank: send_counts[r][d] = sum of hist[r][lo_d..hi_d],
+// send_displs = exclusive prefix over d.
+__global__ void bucket_counts_kernel(const uint32_t* hist, int P, int nb, const int32_t* lo,
+                                     const int32_t* hi, int32_t* scounts, int32_t* sdispls) {
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= P) return;
+    const uint32_t* h = hist + (int64_t)r * nb;
+    int64_t carry = 0;
+    for (int d0 = 0; d0 < P; d0 += kWave) {
+        const int d = d0 + lane();
+        int64_t c = 0;
+        if (d < P)
+            for (int b = lo[d]; b <= hi[d]; b++) c += h[b];
+        int64_t inc = wave_incl_scan(c);
+        if (d < P) {
+            scounts[(int64_t)r * P + d] = (int32_t)c;
+            sdispls[(int64_t)r * P + d] = (int32_t)(carry + inc - c);
+        }
+        carry += __shfl(inc, 63);
+    }
+}
+
+// Stable bucketing: 4 ranks per 256-thread block, LDS bucket pointers per wave.
+__global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* keys, int P, int64_t n,
+                                                            int shift, int nb, int bits,
+                                                            const uint32_t* hist, uint32_t* out) {
+    extern __shared__ uint32_t sptr[];
+    const int w = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + w;
+    uint32_t* ptr = sptr + w * nb;
+    if (r < P) {
+        // exclusive prefix of this rank's histogram into the wave's LDS pointers
+        const uint32_t* h = hist + (int64_t)r * nb;
+        uint32_t carry = 0;
+        for (int b0 = 0; b0 < nb; b0 += kWave) {
+            const int b = b0 + lane();
+            uint32_t c = b < nb ? h[b] : 0;
+            uint32_t inc = (uint32_t)wave_incl_scan(c);
+            if (b < nb) ptr[b] = carry + inc - c;
+            carry += (uint32_t)__shfl((int)inc, 63);
+        }
+    }
+    __syncthreads();
+    if (r >= P) return;
+    const uint32_t* k = keys + (int64_t)r * n;
+    uint32_t* o = out + (int64_t)r * n;
+    const uint64_t lt = (lane() == 0) ? 0ull : (~0ull >> (64 - lane()));
+    for (int64_t base = 0; base < n; base += kWave) {
+        const int64_t i = base + lane();
+        const bool valid = i < n;
+        const uint32_t key = valid ? k[i] : 0u;
+        const uint32_t b = valid ? key >> shift : 0u;
+        uint64_t match = ballot(valid);
+        for (int t = 0; t < bits; t++) {
+            const uint64_t m = ballot(valid && ((b >> t) & 1));
+            match &= ((b >> t) & 1) ? m : ~m;
+        }
+        const uint32_t before = (uint32_t)__popcll(match & lt);
+        const uint32_t p = valid ? ptr[b] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            o[p + before] = key;
+            // the highest lane of each equal-bucket group advances the pointer
+            if ((match >> lane()) == 1ull) ptr[b] = p + (uint32_t)__popcll(match);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+}  // namespace gpumpi

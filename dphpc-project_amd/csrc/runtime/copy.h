@@ -3,10 +3,10 @@
 // The reference copies a whole message with device memcpy on one thread
 // (gpu_libc/cuda_mpi.cu:168, 323; 0.003 GB/s per transfer on V100,
 // benchmarks/results_single_gpu_V100.txt:248).  Here all 64 lanes of the rank's
-// wavefront move 16-byte vectors: the destination is aligned to 16 bytes, the source
-// is read with dword-aligned 16-byte loads (or rebuilt with byte funnel shifts when it
-// is not dword aligned), loads are issued U vectors ahead of the stores, and large
-// payloads use non-temporal stores so a streaming copy does not evict L2.
+// wavefront move 16-byte vectors: the destination is aligned to 16 bytes, the source is
+// read with unaligned 16-byte global loads, U vectors per lane are loaded before any is
+// stored, and large payloads use non-temporal stores so a streaming copy does not evict
+// L2.
 #pragma once
 
 #include "common.h"
@@ -15,36 +15,26 @@ namespace gpumpi {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef u32x4 u32x4_a4 __attribute__((aligned(4)));
+typedef u32x4 u32x4_a1 __attribute__((aligned(1)));
+
+// Global-address-space views: the loads/stores become global_* (not flat_*) instructions.
+#define GPUMPI_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ u32x4 gload16(uint64_t a) {
+    return *(const GPUMPI_GLOBAL u32x4_a1*)(uintptr_t)a;
+}
+template <bool NT>
+__device__ __forceinline__ void gstore16(uint64_t a, u32x4 v) {
+    GPUMPI_GLOBAL u32x4* p = (GPUMPI_GLOBAL u32x4*)(uintptr_t)a;
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 // Payloads at or above this size bypass L2 on the write side.
 constexpr uint64_t kNontemporalBytes = 1u << 20;
 
-__device__ __forceinline__ u32x4 load16_a4(const char* p) {
-    return *(const u32x4_a4*)p;
-}
-__device__ __forceinline__ void store16(char* p, u32x4 v, bool nt) {
-    if (nt) __builtin_nontemporal_store(v, (u32x4*)p);
-    else *(u32x4*)p = v;
-}
-
-// Loads 16 bytes starting at an arbitrary byte address s from dword-aligned words:
-// (s & ~3) .. +20.  The extra dword always holds at least one wanted byte, so the
-// read never leaves the dwords that the wanted bytes occupy.
-__device__ __forceinline__ u32x4 load16_shift(const char* s) {
-    const uint32_t* a = (const uint32_t*)((uintptr_t)s & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)((uintptr_t)s & 3) * 8u;
-    u32x4 w = *(const u32x4_a4*)a;
-    uint32_t w4 = a[4];
-    u32x4 r;
-    r.x = __builtin_amdgcn_alignbit(w.y, w.x, sh);
-    r.y = __builtin_amdgcn_alignbit(w.z, w.y, sh);
-    r.z = __builtin_amdgcn_alignbit(w.w, w.z, sh);
-    r.w = __builtin_amdgcn_alignbit(w4, w.w, sh);
-    return r;
-}
-
-// Copy n bytes from global (or this wave's private) memory to global memory.
-// All 64 lanes must call with identical arguments.
+// Copy n bytes between global buffers (any byte alignment; the destination side is
+// aligned to 16 bytes, the source read with unaligned 16-byte loads, which gfx9 global
+// memory supports).  All 64 lanes must call with identical arguments.
 template <int U = 4>
 __device__ void wave_copy_global(char* d, const char* s, uint64_t n, bool nt) {
     const int l = lane();
@@ -53,18 +43,26 @@ __device__ void wave_copy_global(char* d, const char* s, uint64_t n, bool nt) {
     if ((uint64_t)l < head) d[l] = s[l];
     d += head; s += head; n -= head;
     const uint64_t nvec = n >> 4;
-    const bool a4 = (((uintptr_t)s) & 3) == 0;
+    const uint64_t sa = (uint64_t)(uintptr_t)s, dst = (uint64_t)(uintptr_t)d;
     for (uint64_t base = 0; base < nvec; base += (uint64_t)kWave * U) {
         u32x4 v[U];
 #pragma unroll
         for (int k = 0; k < U; k++) {
             uint64_t i = base + (uint64_t)k * kWave + l;
-            if (i < nvec) v[k] = a4 ? load16_a4(s + 16 * i) : load16_shift(s + 16 * i);
+            if (i < nvec) v[k] = gload16(sa + 16 * i);
         }
+        if (nt) {
 #pragma unroll
-        for (int k = 0; k < U; k++) {
-            uint64_t i = base + (uint64_t)k * kWave + l;
-            if (i < nvec) store16(d + 16 * i, v[k], nt);
+            for (int k = 0; k < U; k++) {
+                uint64_t i = base + (uint64_t)k * kWave + l;
+                if (i < nvec) gstore16<true>(dst + 16 * i, v[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                uint64_t i = base + (uint64_t)k * kWave + l;
+                if (i < nvec) gstore16<false>(dst + 16 * i, v[k]);
+            }
         }
     }
     const uint64_t tail = n & 15;

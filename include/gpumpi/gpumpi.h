@@ -131,10 +131,32 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* plan, int32_t steps_per_graph, vo
 /* Sweep launch shapes on the live buffers, keep the fastest; returns the chosen shape. */
 int gpumpi_a2av_autotune(gpumpi_a2av_plan* plan, void* stream, gpumpi_a2av_tune* chosen,
                          double* best_us);
+/* Phase timestamps of the step kernel (profiling): enable = 1 arms and resets them,
+ * 0 disarms, -1 reads us[3] = {phase 1 done, grid barrier passed, step done} in
+ * microseconds since the first workgroup started (maxima over the last armed launch). */
+int gpumpi_a2av_phase_times(gpumpi_a2av_plan* plan, int32_t enable, double* us);
 /* Launch shape in use and the number of workgroups of the persistent grid. */
 int gpumpi_a2av_get_tune(gpumpi_a2av_plan* plan, gpumpi_a2av_tune* tune, int32_t* grid);
 /* Total payload bytes moved by the last completed step (reads them back from the device). */
 int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* plan, int64_t* bytes);
+
+/* ------------------------------------------------------------------------------------
+ * Bucket-sort caller of the exchange (the NPB IS rank() shape,
+ * toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI/IS/is.c:631-781), on device buffers.
+ * ------------------------------------------------------------------------------------ */
+/* keys[P][n]: dist 0 = uniform [0, 2^key_bits) from Philox (seed, rank, step);
+ * 1 = Zipf by inverse CDF over cdf[ncdf] (device doubles); 2 = NPB IS create_seq keys
+ * of P ranks with max_key_log2 = key_bits (is.c:520-537, seeds is.c:470-510). */
+int gpumpi_gen_keys(uint32_t* keys, int32_t P, int64_t n, uint64_t seed, uint32_t step,
+                    int32_t dist, int32_t key_bits, const double* cdf, int64_t ncdf, void* stream);
+/* NPB-style plan: per-rank bucket histograms, global totals, splitters (is.c:721-751),
+ * send_counts/send_displs [P][P] and the stably bucketed send buffers key_buff1[P][n];
+ * bucket_lo/bucket_hi[P] are each destination's bucket range (empty when lo > hi).
+ * Synchronous on `stream`. */
+int gpumpi_bucket_plan(const uint32_t* keys, int32_t P, int64_t n, int32_t key_bits,
+                       int32_t bucket_bits, uint32_t* key_buff1, int32_t* send_counts,
+                       int32_t* send_displs, int32_t* bucket_lo, int32_t* bucket_hi,
+                       void* stream);
 
 #ifdef __cplusplus
 }

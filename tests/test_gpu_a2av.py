@@ -142,3 +142,72 @@ def test_bucket_exchange_small(dist, pkg, oracle, torch_gpu):
         allk.append(np.sort(got[d].view(np.uint32)))
     # global order after the local sort equals a plain sort of all keys
     assert np.array_equal(np.concatenate(allk), np.sort(keys.ravel()))
+
+
+@pytest.mark.parametrize("dist", ["uniform", "zipf", "npb"])
+def test_gen_keys_and_bucket_plan_match_oracle(dist, pkg, oracle, torch_gpu):
+    """Device key generators and the NPB-style bucket plan are bit-exact with the oracle."""
+    import ctypes
+    from oracle_py import npb_class, ptr
+    torch = torch_gpu
+    print(f"[gpu] gen_keys + bucket_plan {dist}", flush=True)
+    P, n = 16, 4096
+    key_bits, bucket_bits = (11, 9) if dist == "npb" else (23, 10)
+    dev = torch.device("cuda")
+    keys = torch.empty(P * n, dtype=torch.int32, device=dev)
+    cdf_np = zipf_cdf(1 << key_bits) if dist == "zipf" else None
+    cdf = torch.from_numpy(cdf_np).to(dev) if cdf_np is not None else None
+    pkg.gen_keys(keys.data_ptr(), P, n, 0x5eed, 0, dist, key_bits,
+                 cdf.data_ptr() if cdf is not None else 0, cdf.numel() if cdf is not None else 0)
+    torch.cuda.synchronize()
+    got = keys.cpu().numpy().view(np.uint32).reshape(P, n)
+    if dist == "uniform":
+        exp = np.stack([keys_uniform(oracle, 0x5eed, r, n, key_bits) for r in range(P)])
+    elif dist == "zipf":
+        exp = np.stack([keys_zipf(oracle, 0x5eed, r, n, cdf_np) for r in range(P)])
+    else:
+        c = npb_class(oracle, "S")
+        exp = np.zeros((P, n), dtype=np.int32)
+        for r in range(P):
+            row = np.zeros(n, dtype=np.int32)
+            oracle.orc_npb_create_keys(ctypes.byref(c), P, r, ptr(row))
+            exp[r] = row
+        exp = exp.view(np.uint32)
+    assert np.array_equal(got, exp)
+    kb1 = torch.empty_like(keys)
+    sc = torch.zeros((P, P), dtype=torch.int32, device=dev)
+    sd = torch.zeros_like(sc)
+    lo = torch.zeros(P, dtype=torch.int32, device=dev)
+    hi = torch.zeros_like(lo)
+    pkg.bucket_plan(keys.data_ptr(), P, n, key_bits, bucket_bits, kb1.data_ptr(), sc.data_ptr(),
+                    sd.data_ptr(), lo.data_ptr(), hi.data_ptr())
+    e_kb1, e_sc, e_sd, e_lo, e_hi = bucket_plan(oracle, exp, key_bits, bucket_bits)
+    assert np.array_equal(kb1.cpu().numpy().view(np.uint32).reshape(P, n), e_kb1)
+    assert np.array_equal(sc.cpu().numpy(), e_sc)
+    assert np.array_equal(sd.cpu().numpy(), e_sd)
+    assert np.array_equal(lo.cpu().numpy(), e_lo) and np.array_equal(hi.cpu().numpy(), e_hi)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "zipf"])
+def test_full_size_exchange(dist, pkg, oracle, torch_gpu):
+    """512 ranks x 2^18 keys: every destination bit-exact against the oracle's Alltoallv."""
+    import importlib.util, os
+    torch = torch_gpu
+    print(f"[gpu] full-size exchange {dist}", flush=True)
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(os.path.dirname(__file__), "..", "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    ex = bench.Exchange(pkg, torch, 512, 1 << 18, dist)
+    plan = ex.plan(pkg)
+    plan.graph_replay(2)
+    torch.cuda.synchronize()
+    P, n = ex.P, ex.n
+    kb1 = ex.kb1.cpu().numpy().view(np.uint32).reshape(P, n)
+    send = [kb1[s].view(np.uint8) for s in range(P)]
+    exp, erc, erd = alltoallv(oracle, send, ex.counts.astype(np.int32), ex.sd.cpu().numpy(), 4, ex.rtot)
+    recv = ex.recv.cpu().numpy().view(np.uint8)
+    assert np.array_equal(ex.rc.cpu().numpy(), erc) and np.array_equal(ex.rd.cpu().numpy(), erd)
+    for d in range(P):
+        o = int(ex.roff[d]) * 4
+        assert np.array_equal(recv[o:o + int(ex.rtot[d]) * 4], exp[d][:int(ex.rtot[d]) * 4]), d
+    plan.close()

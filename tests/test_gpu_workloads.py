@@ -1,0 +1,176 @@
+"""Rank-grid workload programs against the oracle (bit-exact for integer, byte and
+index results; FP64 reductions bit-exact because the device folds in the reference's
+rank order 0..P-1, gpu_mpi/mpi.cu:209-225)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle_py import fill_bytes, fill_int, fill_normal, reduce
+
+pytestmark = pytest.mark.gpu
+
+MPI_INT, MPI_DOUBLE = 2, 13
+OP_MAX, OP_SUM = 0, 2
+
+
+def test_mcpi(pkg, oracle, torch_gpu):
+    """Config 1 shape: 64 ranks, 2^20 pairs each; int64 count bit-exact."""
+    from dphpc_project_amd import programs as pr
+    torch = torch_gpu
+    P, n, seed = 64, 1 << 20, 0x5eed
+    print(f"[gpu] mcpi P={P} n={n}", flush=True)
+    per = torch.zeros(P, dtype=torch.int64, device="cuda")
+    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=20.0) as rt:
+        rt.run("mcpi", pr.McpiArgs(seed, n, per.data_ptr(), tot.data_ptr(), sec.data_ptr()))
+    got = per.cpu().numpy()
+    # oracle on a subset of ranks at full n, all ranks at small n below
+    for r in (0, 1, 31, 63):
+        assert got[r] == oracle.orc_mcpi_hits(ctypes.c_uint64(seed), r, ctypes.c_int64(n)), r
+    assert tot.item() == int(got.sum())
+    est = 4.0 * tot.item() / (P * n)
+    assert abs(est - np.pi) < 1e-3
+
+
+def test_mcpi_all_ranks_small(pkg, oracle, torch_gpu):
+    from dphpc_project_amd import programs as pr
+    torch = torch_gpu
+    P, n, seed = 64, 4096, 0xabcdef
+    per = torch.zeros(P, dtype=torch.int64, device="cuda")
+    tot = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=64 << 20) as rt:
+        rt.run("mcpi", pr.McpiArgs(seed, n, per.data_ptr(), tot.data_ptr(), sec.data_ptr()))
+    exp = [oracle.orc_mcpi_hits(ctypes.c_uint64(seed), r, ctypes.c_int64(n)) for r in range(P)]
+    assert per.cpu().tolist() == exp
+    assert tot.item() == sum(exp)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("nbytes", [8, 80, 81, 4096, 1 << 20])
+def test_p2p_payloads(mode, nbytes, pkg, oracle, torch_gpu):
+    """Config 2 shapes at reduced size: ping-pong / ring, blocking / non-blocking; every
+    received byte equal to the sender's Philox payload (and to the oracle's bytes)."""
+    from dphpc_project_amd import programs as pr
+    torch = torch_gpu
+    P, seed = 16, 0x77
+    print(f"[gpu] p2p mode={mode} bytes={nbytes}", flush=True)
+    sb = torch.zeros(P * nbytes, dtype=torch.uint8, device="cuda")
+    rb = torch.zeros(P * nbytes, dtype=torch.uint8, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    mis = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=20.0) as rt:
+        rt.run("p2p", pr.P2pArgs(seed, mode, 3, 1, 0, nbytes, sb.data_ptr(), rb.data_ptr(),
+                                 sec.data_ptr(), mis.data_ptr()))
+    assert mis.cpu().tolist() == [0] * P
+    recv = rb.cpu().numpy().reshape(P, nbytes)
+    send = sb.cpu().numpy().reshape(P, nbytes)
+    for r in range(P):
+        assert np.array_equal(send[r], fill_bytes(oracle, seed, r, nbytes))
+        origin = (r if r % 2 == 0 else r ^ 1) if mode < 2 else (r - 1) % P
+        assert np.array_equal(recv[r], fill_bytes(oracle, seed, origin, nbytes)), r
+
+
+def run_coll(pkg, torch, P, kind, dtype, op, count, root=0, seed=0x1234):
+    from dphpc_project_amd import programs as pr
+    es = 8 if dtype == MPI_DOUBLE else 4
+    in_elems = count * (P if kind == "scatter" else 1)
+    out_elems = count * (P if kind in ("allgather", "gather") else 1)
+    inb = torch.zeros(P * in_elems * es + 16, dtype=torch.uint8, device="cuda")
+    outb = torch.zeros(P * out_elems * es + 16, dtype=torch.uint8, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    prog_op = 0 if op == OP_SUM else 1  # the program's own encoding: 0 = MPI_SUM, 1 = MPI_MAX
+    args = pr.CollArgs(seed, pr.COLL_KINDS[kind], dtype, prog_op, root, count, 1, 0,
+                       inb.data_ptr(), outb.data_ptr(), in_elems * es, out_elems * es, sec.data_ptr())
+    with pkg.Runtime(P, heap_bytes=128 << 20, timeout_s=20.0) as rt:
+        rt.run("coll", args)
+    npdt = np.float64 if dtype == MPI_DOUBLE else np.int32
+    out = outb.cpu().numpy()[:P * out_elems * es].view(npdt).reshape(P, out_elems)
+    return out
+
+
+def gen(oracle, dtype, seed, r, n):
+    return fill_normal(oracle, seed, r, n) if dtype == MPI_DOUBLE else fill_int(oracle, seed, r, n)
+
+
+@pytest.mark.parametrize("dtype", [MPI_INT, MPI_DOUBLE])
+@pytest.mark.parametrize("op", [OP_SUM, OP_MAX])
+@pytest.mark.parametrize("P,count", [(7, 1), (64, 1000), (33, 4097)])
+def test_reduce_allreduce(dtype, op, P, count, pkg, oracle, torch_gpu):
+    """Config 3: Reduce and Allreduce bit-exact with the oracle in reference fold order."""
+    print(f"[gpu] reduce/allreduce dtype={dtype} op={op} P={P} count={count}", flush=True)
+    seed = 0x1234
+    ins = [gen(oracle, dtype, seed, r, count) for r in range(P)]
+    rc, exp = reduce(oracle, dtype, op, ins)
+    assert rc == 0
+    out = run_coll(pkg, torch_gpu, P, "reduce", dtype, op, count, root=P // 2, seed=seed)
+    assert np.array_equal(out[P // 2], exp)
+    out = run_coll(pkg, torch_gpu, P, "allreduce", dtype, op, count, seed=seed)
+    for r in range(P):
+        assert np.array_equal(out[r], exp), r
+    if dtype == MPI_DOUBLE and op == OP_SUM:
+        # against a different (pairwise) order: within 1e-12 normwise
+        ref = np.sum(np.stack(ins), axis=0)
+        norm = np.sum(np.abs(np.stack(ins)), axis=0)
+        assert np.all(np.abs(out[0] - ref) <= 1e-12 * norm)
+
+
+@pytest.mark.parametrize("dtype", [MPI_INT, MPI_DOUBLE])
+@pytest.mark.parametrize("P,count", [(5, 3), (64, 256), (128, 1000)])
+def test_bcast_alltoall_gather_scatter(dtype, P, count, pkg, oracle, torch_gpu):
+    print(f"[gpu] bcast/alltoall/gather/scatter dtype={dtype} P={P} count={count}", flush=True)
+    seed = 0x99
+    ins = [gen(oracle, dtype, seed, r, count) for r in range(P)]
+    root = P - 1
+    out = run_coll(pkg, torch_gpu, P, "bcast", dtype, OP_SUM, count, root=root, seed=seed)
+    for r in range(P):
+        assert np.array_equal(out[r], ins[root])
+    # Alltoall: per-rank total `count`, per pair count // P
+    per = count // P
+    if per > 0:
+        out = run_coll(pkg, torch_gpu, P, "alltoall", dtype, OP_SUM, count, seed=seed)
+        for d in range(P):
+            exp = np.concatenate([ins[s][d * per:(d + 1) * per] for s in range(P)])
+            assert np.array_equal(out[d][:P * per], exp)
+    out = run_coll(pkg, torch_gpu, P, "allgather", dtype, OP_SUM, count, seed=seed)
+    exp = np.concatenate(ins)
+    for r in range(P):
+        assert np.array_equal(out[r], exp)
+    out = run_coll(pkg, torch_gpu, P, "gather", dtype, OP_SUM, count, root=1 % P, seed=seed)
+    assert np.array_equal(out[1 % P], exp)
+    out = run_coll(pkg, torch_gpu, P, "scatter", dtype, OP_SUM, count, root=0, seed=seed)
+    src = gen(oracle, dtype, seed, 0, count * P)
+    for r in range(P):
+        assert np.array_equal(out[r], src[r * count:(r + 1) * count])
+
+
+def test_heap_churn(pkg, oracle, torch_gpu):
+    """Config 3 allocator churn at reduced length: stamps survive, live blocks are disjoint
+    and aligned, the heap returns to empty, with an Allreduce interleaved."""
+    from dphpc_project_amd import programs as pr
+    from oracle_py import ptr
+    torch = torch_gpu
+    P, ops = 256, 20000
+    print(f"[gpu] heap churn P={P} ops={ops}", flush=True)
+    err = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    peak = torch.zeros(P, dtype=torch.int64, device="cuda")
+    lp = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    ls = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    heap = 1 << 30
+    with pkg.Runtime(P, heap_bytes=heap, timeout_s=60.0) as rt:
+        rt.run("churn", pr.ChurnArgs(0x5eed, ops, 1000, 1, err.data_ptr(), peak.data_ptr(),
+                                     lp.data_ptr(), ls.data_ptr(), sec.data_ptr()))
+        stats = rt.heap_stats()
+    assert err.cpu().tolist() == [0] * P
+    assert stats["in_use"] == 0
+    p = lp.cpu().numpy().view(np.uint64)
+    s = ls.cpu().numpy().view(np.uint64)
+    keep = p != 0
+    p, s = np.ascontiguousarray(p[keep]), np.ascontiguousarray(s[keep])
+    base = int(p.min()) & ~0xFFFF
+    assert oracle.orc_heap_check(ptr(p), ptr(s), ctypes.c_int64(p.size), ctypes.c_uint64(base),
+                                 ctypes.c_uint64(heap + (1 << 20))) == 0
+    assert 0 < peak.cpu().max() <= 64
