@@ -12,8 +12,12 @@ value  = algorithmic bytes / time = 2 x payload bytes (read once + written once)
          step / seconds per step, in GB/s.
 Also reported: the plain per-call path (one rank-grid kernel launch per MPI call:
 MPI_Alltoall then MPI_Alltoallv on 512 wavefront ranks) and the ratio, the Zipf(1.1)
-variant, and the host-MPI CPU baseline (the same exchange under MPICH, one rank per
-host core, 2^18 keys per rank).
+variant, the host-MPI CPU baseline (the same exchange under MPICH, one rank per host
+core, 2^18 keys per rank), and served MPI jobs: Poisson arrivals of independent dot /
+Alltoall jobs of 8..256 ranks at fractions of the plain path's capacity, p50/p99
+latency (arrival -> result on the host) and jobs/s for the device-resident server
+versus one plain rank-grid launch per job; every served result is compared with the
+same job run alone on the plain path.
 """
 import argparse
 import ctypes
@@ -161,6 +165,114 @@ def cpu_baseline(args):
                       f"{args.cpu_iters} timed repetitions, sorted_ok={kv['sorted_ok']}"}
 
 
+def make_jobs(pkg, njobs, seed, heap_budget):
+    """Config-5 job stream: ranks log-uniform 8..256, dot (lognormal FP64 length in
+    [2^10, 2^22]) or Alltoall (log-uniform 1 KiB .. min(16 MiB, H/(2 P^2)) per pair)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(njobs):
+        P = int(2 ** (3 + 5 * rng.random()))
+        P = min(P, 256)
+        if rng.random() < 0.5:
+            n = int(np.clip(np.exp(rng.normal(np.log(1 << 16), 1.5)), 1 << 10, 1 << 22))
+            jobs.append(pkg.Job(i, 7000 + i, 0, P, n))
+        else:
+            cap = max(1024, min(16 << 20, heap_budget // (8 * P * P)))
+            m = int(2 ** rng.uniform(10, np.log2(cap)))
+            jobs.append(pkg.Job(i, 7000 + i, 1, P, m))
+    return jobs
+
+
+def serve_latency(pkg, torch, args):
+    """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
+    the host, for the device-resident server and for one plain rank-grid launch per job."""
+    import numpy as np
+    heap = 16 << 30
+    budget = int(0.9 * heap)
+    jobs = make_jobs(pkg, args.serve_jobs, 5, budget)
+    # plain path capacity: the jobs back to back
+    rt = pkg.Runtime(256, heap_bytes=heap, timeout_s=60.0)
+    res = torch.zeros(len(jobs) * 64, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    for j in jobs[:8]:
+        rt.run_job(j, res.data_ptr() + 64 * j.id, stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in jobs:
+        rt.run_job(j, res.data_ptr() + 64 * j.id, stream.cuda_stream)
+    torch.cuda.synchronize()
+    plain_capacity = len(jobs) / (time.perf_counter() - t0)
+    plain_raw = res.cpu().numpy().reshape(len(jobs), 64)
+    out = {"jobs": len(jobs), "plain_capacity_jobs_per_s": round(plain_capacity, 1), "rates": []}
+    srv = pkg.Server(args.serve_slots, queue_capacity=8192, max_job_ranks=256,
+                     heap_fraction=0.9, ranks_per_block=4, heap_bytes=heap, timeout_s=60.0)
+    mismatches = 0
+    for frac in args.serve_rates:
+        lam = frac * plain_capacity
+        rng = np.random.default_rng(int(frac * 1000))
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(jobs)))
+        # ---- served
+        lat_s = {}
+        start = time.perf_counter()
+        i = 0
+        while len(lat_s) < len(jobs):
+            now = time.perf_counter() - start
+            while i < len(jobs) and arrivals[i] <= now:
+                if not srv.submit(jobs[i]):
+                    break
+                i += 1
+            for r in srv.poll():
+                lat_s[r.id] = (time.perf_counter() - start) - arrivals[r.id]
+                p = pkg.JobResult.from_buffer_copy(plain_raw[r.id].tobytes())
+                if r.status != 1 or (r.dot != p.dot) or (r.checksum != p.checksum):
+                    mismatches += 1
+            if now - arrivals[-1] > 120:
+                break
+        served_span = time.perf_counter() - start
+        # ---- plain: each job launched as its own rank grid at its arrival time
+        events = [torch.cuda.Event() for _ in jobs]
+        lat_p = {}
+        start = time.perf_counter()
+        i = 0
+        pending = []
+        while len(lat_p) < len(jobs):
+            now = time.perf_counter() - start
+            while i < len(jobs) and arrivals[i] <= now:
+                rt.run_job(jobs[i], res.data_ptr() + 64 * jobs[i].id, stream.cuda_stream)
+                events[i].record(stream)
+                pending.append(i)
+                i += 1
+            still = []
+            for k in pending:
+                if events[k].query():
+                    lat_p[k] = (time.perf_counter() - start) - arrivals[k]
+                else:
+                    still.append(k)
+            pending = still
+            if now - arrivals[-1] > 120:
+                break
+        plain_span = time.perf_counter() - start
+        ls = np.array(list(lat_s.values())) * 1e3
+        lp = np.array(list(lat_p.values())) * 1e3
+        row = {"rate_fraction": frac, "rate_jobs_per_s": round(lam, 1),
+               "served": {"p50_ms": round(float(np.percentile(ls, 50)), 3),
+                          "p99_ms": round(float(np.percentile(ls, 99)), 3),
+                          "jobs_per_s": round(len(ls) / served_span, 1)},
+               "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lp, 99)), 3),
+                         "jobs_per_s": round(len(lp) / plain_span, 1)}}
+        log(f"serve rate {frac}x ({lam:.0f}/s): served p50 {row['served']['p50_ms']} ms "
+            f"p99 {row['served']['p99_ms']} ms; plain p50 {row['plain']['p50_ms']} ms "
+            f"p99 {row['plain']['p99_ms']} ms")
+        out["rates"].append(row)
+    stats = srv.stop()
+    rt.close()
+    out["server_stats"] = stats
+    out["result_mismatches_vs_plain"] = mismatches
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +286,10 @@ def main():
     ap.add_argument("--no-zipf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=400)
+    ap.add_argument("--serve-slots", type=int, default=1024)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 2.0])
     ap.add_argument("--tune", type=str, default="", help="block_threads,blocks_per_cu,lines,nt")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (scale for rocprof FETCH/WRITE_SIZE)")
@@ -261,6 +377,8 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
 
+    serve = None if args.no_serve else serve_latency(pkg, torch, args)
+
     cpu = None if args.no_cpu else cpu_baseline(args)
 
     traffic = None
@@ -295,6 +413,7 @@ def main():
         "plain_path": plain,
         "batched_vs_plain": round(plain["ms_per_step"] / ms, 2) if plain else None,
         "zipf": zipf,
+        "serving": serve,
     }
     print(json.dumps(result), flush=True)
 

@@ -45,6 +45,18 @@ class Config(ctypes.Structure):
     ]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("type", ctypes.c_int32),
+                ("ranks", ctypes.c_int32), ("size", ctypes.c_int64)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("ranks", ctypes.c_int32),
+                ("dot", ctypes.c_double), ("checksum", ctypes.c_uint64),
+                ("t_admit_ticks", ctypes.c_uint64), ("t_done_ticks", ctypes.c_uint64),
+                ("bytes", ctypes.c_int64)]
+
+
 class A2avTune(ctypes.Structure):
     _fields_ = [
         ("block_threads", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
@@ -84,6 +96,13 @@ _SIGNATURES = {
     "gpumpi_a2av_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_get_tune": (ctypes.c_int, [_P, ctypes.POINTER(A2avTune), ctypes.POINTER(_I32)]),
     "gpumpi_a2av_payload_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "gpumpi_serve_start": (ctypes.c_int, [ctypes.POINTER(Config), _I32, _I32, ctypes.c_double,
+                                          ctypes.POINTER(_P)]),
+    "gpumpi_serve_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job)]),
+    "gpumpi_serve_poll": (ctypes.c_int, [_P, ctypes.POINTER(JobResult), _I32]),
+    "gpumpi_serve_stop": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "gpumpi_run_job": (ctypes.c_int, [_P, ctypes.POINTER(Job), _P, _P]),
+    "gpumpi_job_heap_bytes": (ctypes.c_int64, [ctypes.POINTER(Job)]),
     "gpumpi_gen_keys": (ctypes.c_int, [_P, _I32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
                                        _I32, _I32, _P, ctypes.c_int64, _P]),
     "gpumpi_bucket_plan": (ctypes.c_int, [_P, _I32, ctypes.c_int64, _I32, _I32, _P, _P, _P, _P, _P, _P]),
@@ -211,6 +230,13 @@ class Runtime:
         self.lib.gpumpi_last_device_error(self.h, ctypes.byref(code), ctypes.byref(rank))
         return code.value, rank.value
 
+    def run_job(self, job, result_ptr, stream=0):
+        """Plain path: launch one job as its own rank grid (asynchronous)."""
+        rc = self.lib.gpumpi_run_job(self.h, ctypes.byref(job), result_ptr,
+                                     ctypes.c_void_p(stream) if stream else None)
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_run_job failed ({rc}): {self.last_error()}")
+
     def heap_stats(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         self.lib.gpumpi_heap_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
@@ -232,6 +258,59 @@ class Runtime:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Device-resident MPI job server (gpumpi_serve_*)."""
+
+    def __init__(self, slots, queue_capacity=4096, max_job_ranks=256, heap_fraction=0.9, **cfg):
+        self.lib = load_library()
+        self.cfg = default_config(num_ranks=slots, **cfg)
+        h = _P()
+        rc = self.lib.gpumpi_serve_start(ctypes.byref(self.cfg), queue_capacity, max_job_ranks,
+                                         heap_fraction, ctypes.byref(h))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_serve_start failed ({rc}): "
+                              f"{self.lib.gpumpi_last_error(None).decode()}")
+        self.h = h
+        self._buf = (JobResult * 256)()
+
+    def submit(self, job):
+        rc = self.lib.gpumpi_serve_submit(self.h, ctypes.byref(job))
+        if rc == E_NOMEM:
+            return False
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_serve_submit failed ({rc})")
+        return True
+
+    def poll(self):
+        out = []
+        while True:
+            n = self.lib.gpumpi_serve_poll(self.h, self._buf, len(self._buf))
+            out.extend(JobResult.from_buffer_copy(self._buf[i]) for i in range(n))
+            if n < len(self._buf):
+                return out
+
+    def stop(self):
+        if not self.h:
+            return None
+        stats = (ctypes.c_int64 * 4)()
+        rc = self.lib.gpumpi_serve_stop(self.h, stats)
+        self.h = None
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_serve_stop: {rc}: {self.lib.gpumpi_last_error(None).decode()}")
+        return {"admitted": stats[0], "rejected": stats[1], "max_concurrent": stats[2],
+                "backfilled": stats[3]}
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
+
+
+def job_heap_bytes(job):
+    return load_library().gpumpi_job_heap_bytes(ctypes.byref(job))
 
 
 class A2avPlan:

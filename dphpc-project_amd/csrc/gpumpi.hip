@@ -106,6 +106,22 @@ __global__ void clear_status_kernel(Runtime* R) {
     }
 }
 
+// Launch-local world size (plain one-launch-per-job mode): the first k ranks of the
+// runtime form MPI_COMM_WORLD; collective sequence numbers and barrier words restart.
+__global__ void resize_world_kernel(Runtime* R, int k) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < k; r += gridDim.x * blockDim.x) {
+        R->world->slots[r].seq = 0;
+        R->world->seqs[r] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        R->nranks = k;
+        R->world->size = k;
+        R->world_group->size = k;
+        R->world->arrive.value = 0;
+        R->world->gen.value = 0;
+    }
+}
+
 struct ProgramEntry {
     const char* name;
     const void* kernel;
@@ -136,6 +152,7 @@ static const ProgramEntry kPrograms[] = {
     GPUMPI_PROGRAM("npb_is", progs::w_npb_is),
     GPUMPI_PROGRAM("particles", progs::w_particles),
     GPUMPI_PROGRAM("job", progs::w_job),
+    GPUMPI_PROGRAM("serve", progs::w_serve),
 };
 #undef GPUMPI_PROGRAM
 constexpr int kNumPrograms = (int)(sizeof(kPrograms) / sizeof(kPrograms[0]));
@@ -158,6 +175,7 @@ struct gpumpi_runtime {
     hipStream_t own_stream = nullptr;
     std::string err;
     bool needs_reset = false;
+    int world_size = 0;  // ranks of MPI_COMM_WORLD as last set on the device
 };
 
 static thread_local std::string g_host_error;
@@ -341,6 +359,7 @@ int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
         gpumpi_runtime_destroy(rt);
         return rc;
     }
+    rt->world_size = cfg.num_ranks;
     *out = rt;
     return GPUMPI_OK;
 }
@@ -387,6 +406,12 @@ static int stage_args(gpumpi_runtime* rt, const void* args, size_t bytes, hipStr
 
 static int launch_program(gpumpi_runtime* rt, const ProgramEntry* pe, int argc, char** d_argv,
                           hipStream_t s) {
+    if (rt->world_size != rt->cfg.num_ranks) {
+        // a plain per-job launch shrank MPI_COMM_WORLD; restore the full rank grid
+        hipLaunchKernelGGL(resize_world_kernel, dim3(4), dim3(256), 0, s, rt->d_rt,
+                           rt->cfg.num_ranks);
+        rt->world_size = rt->cfg.num_ranks;
+    }
     const int rpb = rt->cfg.ranks_per_block;
     const int block = 64 * rpb;
     const int grid = (rt->cfg.num_ranks + rpb - 1) / rpb;
@@ -914,6 +939,177 @@ int gpumpi_bucket_plan(const uint32_t* keys, int32_t P, int64_t n, int32_t key_b
     hipFree(hist);
     hipFree(totals);
     return rc;
+}
+
+}  // extern "C"
+
+// ==================================================================== served jobs
+struct gpumpi_server {
+    gpumpi_runtime* rt = nullptr;
+    progs::ServeShared sh{};
+    progs::ServeShared* d_sh = nullptr;
+    std::vector<void*> dev_allocs;
+    std::vector<void*> host_allocs;
+    hipStream_t stream = nullptr;
+    uint64_t submitted = 0;
+    uint64_t polled = 0;
+    int32_t qcap = 0;
+    bool running = false;
+};
+
+extern "C" {
+
+int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t max_job_ranks,
+                       double heap_fraction, gpumpi_server** out) {
+    if (!cfg || !out || queue_capacity < 1 || max_job_ranks < 1 || cfg->num_ranks < 2 ||
+        heap_fraction <= 0 || heap_fraction > 1)
+        return GPUMPI_E_ARG;
+    *out = nullptr;
+    gpumpi_server* sv = new gpumpi_server();
+    int rc = gpumpi_runtime_create(cfg, &sv->rt);
+    if (rc != GPUMPI_OK) {
+        delete sv;
+        return rc;
+    }
+    const int S = cfg->num_ranks;
+    const int jcap = S;  // at most one job per worker slot
+    sv->qcap = queue_capacity;
+    auto halloc = [&](void** p, size_t b) {
+        if (hipHostMalloc(p, b, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return false;
+        std::memset(*p, 0, b);
+        sv->host_allocs.push_back(*p);
+        return true;
+    };
+    auto dalloc = [&](void** p, size_t b) {
+        if (hipMalloc(p, b) != hipSuccess) return false;
+        sv->dev_allocs.push_back(*p);
+        return hipMemset(*p, 0, b) == hipSuccess;
+    };
+    progs::ServeShared& h = sv->sh;
+    bool ok = halloc((void**)&h.q, sizeof(progs::JobDesc) * queue_capacity) &&
+              halloc((void**)&h.q_tail, 64) &&
+              halloc((void**)&h.done, sizeof(progs::JobResult) * queue_capacity) &&
+              halloc((void**)&h.done_count, 64) && halloc((void**)&h.stop, 64) &&
+              dalloc((void**)&h.dq, sizeof(progs::JobDesc) * queue_capacity) &&
+              dalloc((void**)&h.boxes, sizeof(progs::SlotBox) * S) &&
+              dalloc((void**)&h.jobs, sizeof(progs::JobSlot) * jcap) &&
+              dalloc((void**)&h.free_slots, sizeof(int32_t) * S) &&
+              dalloc((void**)&h.slot_lists, sizeof(int32_t) * (size_t)jcap * max_job_ranks) &&
+              dalloc((void**)&h.exit_flag, 64) && dalloc((void**)&h.stats, 64) &&
+              dalloc((void**)&sv->d_sh, sizeof(progs::ServeShared));
+    if (!ok) {
+        gpumpi_serve_stop(sv, nullptr);
+        return GPUMPI_E_NOMEM;
+    }
+    h.qcap = queue_capacity;
+    h.jcap = jcap;
+    h.max_ranks = max_job_ranks;
+    h.lookahead = 16;
+    h.heap_budget = (int64_t)((double)sv->rt->h.heap.bytes * heap_fraction);
+    if (hipMemcpy(sv->d_sh, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamCreate(&sv->stream) != hipSuccess) {
+        gpumpi_serve_stop(sv, nullptr);
+        return GPUMPI_E_HIP;
+    }
+    progs::ServeArgs a{sv->d_sh};
+    rc = gpumpi_run_async(sv->rt, "serve", &a, sizeof(a), sv->stream);
+    if (rc != GPUMPI_OK) {
+        gpumpi_serve_stop(sv, nullptr);
+        return rc;
+    }
+    sv->running = true;
+    *out = sv;
+    return GPUMPI_OK;
+}
+
+int gpumpi_serve_submit(gpumpi_server* sv, const gpumpi_job* job) {
+    if (!sv || !job || !sv->running) return GPUMPI_E_ARG;
+    const uint64_t done = __atomic_load_n(sv->sh.done_count, __ATOMIC_ACQUIRE);
+    if (sv->submitted - done >= (uint64_t)sv->qcap) return GPUMPI_E_NOMEM;  // ring full
+    progs::JobDesc d{job->id, job->seed, job->type, job->ranks, job->size};
+    sv->sh.q[sv->submitted % (uint64_t)sv->qcap] = d;
+    sv->submitted++;
+    __atomic_store_n(sv->sh.q_tail, sv->submitted, __ATOMIC_RELEASE);
+    return GPUMPI_OK;
+}
+
+int gpumpi_serve_poll(gpumpi_server* sv, gpumpi_job_result* out, int32_t max) {
+    if (!sv || (!out && max > 0)) return GPUMPI_E_ARG;
+    const uint64_t done = __atomic_load_n(sv->sh.done_count, __ATOMIC_ACQUIRE);
+    int n = 0;
+    while (sv->polled < done && n < max) {
+        const progs::JobResult& r = sv->sh.done[sv->polled % (uint64_t)sv->qcap];
+        gpumpi_job_result& o = out[n++];
+        o.id = r.id;
+        o.status = r.status;
+        o.ranks = r.ranks;
+        o.dot = r.dot;
+        o.checksum = r.checksum;
+        o.t_admit_ticks = r.t_admit;
+        o.t_done_ticks = r.t_done;
+        o.bytes = r.bytes;
+        sv->polled++;
+    }
+    return n;
+}
+
+int gpumpi_serve_stop(gpumpi_server* sv, int64_t* stats) {
+    if (!sv) return GPUMPI_E_ARG;
+    int rc = GPUMPI_OK;
+    if (sv->running) {
+        __atomic_store_n(sv->sh.stop, 1, __ATOMIC_RELEASE);
+        if (hipStreamSynchronize(sv->stream) != hipSuccess) rc = GPUMPI_E_HIP;
+        if (stats && sv->sh.stats)
+            hipMemcpy(stats, sv->sh.stats, sizeof(int64_t) * 4, hipMemcpyDeviceToHost);
+        Runtime hr;
+        if (hipMemcpy(&hr, sv->rt->d_rt, sizeof(Runtime), hipMemcpyDeviceToHost) == hipSuccess &&
+            hr.abort_flag) {
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "device error %d on rank %d (line %d): %s", hr.err.code,
+                          hr.err.rank, hr.err.line, hr.err.msg);
+            g_host_error = msg;
+            rc = GPUMPI_E_DEVICE;
+        }
+    }
+    if (sv->stream) {
+        hipStreamSynchronize(sv->stream);
+        hipStreamDestroy(sv->stream);
+    }
+    for (void* p : sv->dev_allocs) hipFree(p);
+    for (void* p : sv->host_allocs) hipHostFree(p);
+    if (sv->rt) gpumpi_runtime_destroy(sv->rt);
+    delete sv;
+    return rc;
+}
+
+// Plain mode: one rank-grid launch of `job.ranks` ranks for one job (asynchronous).
+int gpumpi_run_job(gpumpi_runtime* rt, const gpumpi_job* job, gpumpi_job_result* dev_result,
+                   void* stream) {
+    if (!rt || !job || !dev_result || job->ranks < 1 || job->ranks > rt->cfg.num_ranks)
+        return GPUMPI_E_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : rt->own_stream;
+    hipLaunchKernelGGL(resize_world_kernel, dim3(4), dim3(256), 0, s, rt->d_rt, job->ranks);
+    rt->world_size = job->ranks;
+    progs::JobArgs a;
+    a.job = progs::JobDesc{job->id, job->seed, job->type, job->ranks, job->size};
+    a.out = (progs::JobResult*)dev_result;
+    int rc = stage_args(rt, &a, sizeof(a), s);
+    if (rc != GPUMPI_OK) return rc;
+    const ProgramEntry* pe = find_program("job");
+    const int rpb = rt->cfg.ranks_per_block;
+    const int grid = (job->ranks + rpb - 1) / rpb;
+    Runtime* d = rt->d_rt;
+    int argc = 0;
+    char** argv = nullptr;
+    void* kargs[] = {(void*)&d, (void*)&argc, (void*)&argv};
+    HIP_TRY(hipLaunchKernel(pe->kernel, dim3(grid), dim3(64 * rpb), kargs, 0, s));
+    return GPUMPI_OK;
+}
+
+int64_t gpumpi_job_heap_bytes(const gpumpi_job* job) {
+    if (!job) return -1;
+    progs::JobDesc d{job->id, job->seed, job->type, job->ranks, job->size};
+    return progs::job_bytes(d);
 }
 
 }  // extern "C"

@@ -18,7 +18,8 @@
  *   gpumpi_last_error ...... CudaMPI::recordError / printLastError (gpu_libc/assert.cu:24-71)
  *   gpumpi_a2av_*  ......... no reference counterpart: the batched persistent-kernel
  *                            MPI_Alltoallv step (gpu_mpi/mpi.cu:363-408 semantics)
- *   gpumpi_serve_* ......... no reference counterpart: the device-resident job server
+ *   gpumpi_serve_*, gpumpi_run_job ... no reference counterpart: the device-resident job
+ *                            server and the one-launch-per-job plain path
  *
  * A reference-side maintainer binds these with ctypes/cffi or links them directly; the
  * exact bindings are listed in INTEGRATION.md.
@@ -157,6 +158,54 @@ int gpumpi_bucket_plan(const uint32_t* keys, int32_t P, int64_t n, int32_t key_b
                        int32_t bucket_bits, uint32_t* key_buff1, int32_t* send_counts,
                        int32_t* send_displs, int32_t* bucket_lo, int32_t* bucket_hi,
                        void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Served MPI jobs.  A job is an independent MPI program on its own communicator:
+ *   type 0: dot product of two FP64 vectors of `size` elements over `ranks` ranks
+ *           (MPI_Allreduce SUM of the local dots), result in .dot;
+ *   type 1: MPI_Alltoall of `size` bytes per pair, result = checksum of the received
+ *           bytes in .checksum.
+ * gpumpi_serve_*: a persistent rank grid of cfg->num_ranks wavefront slots; slot 0 admits
+ * jobs in arrival order (bounded backfill) when enough slots are idle and the job's
+ * heap need fits in heap_fraction of the device heap; the others run jobs.
+ * gpumpi_run_job: the plain path -- one rank-grid launch per job.
+ * ------------------------------------------------------------------------------------ */
+typedef struct gpumpi_job {
+    uint64_t id;
+    uint64_t seed;
+    int32_t type;
+    int32_t ranks;
+    int64_t size;
+} gpumpi_job;
+
+typedef struct gpumpi_job_result {
+    uint64_t id;
+    int32_t status;             /* 1 done, -1 rejected (never fits), -2 device error */
+    int32_t ranks;
+    double dot;
+    uint64_t checksum;
+    uint64_t t_admit_ticks;     /* device wall clock (100 MHz) at admission */
+    uint64_t t_done_ticks;      /* device wall clock when the last rank finished */
+    int64_t bytes;              /* heap bytes reserved for the job */
+} gpumpi_job_result;
+
+typedef struct gpumpi_server gpumpi_server;
+
+int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t max_job_ranks,
+                       double heap_fraction, gpumpi_server** out);
+/* Non-blocking; GPUMPI_E_NOMEM when queue_capacity jobs are already outstanding. */
+int gpumpi_serve_submit(gpumpi_server* server, const gpumpi_job* job);
+/* Copies up to max new completions; returns how many. */
+int gpumpi_serve_poll(gpumpi_server* server, gpumpi_job_result* out, int32_t max);
+/* Finishes every submitted job, stops the server; stats[4] (optional) = admitted,
+ * rejected, max concurrently running jobs, jobs admitted ahead of an older one. */
+int gpumpi_serve_stop(gpumpi_server* server, int64_t* stats);
+/* Plain path: launch job->ranks ranks of `rt` for one job on `stream` (asynchronous);
+ * the result lands in dev_result (device memory). */
+int gpumpi_run_job(gpumpi_runtime* rt, const gpumpi_job* job, gpumpi_job_result* dev_result,
+                   void* stream);
+/* Heap bytes a job reserves (the admission rule of the server). */
+int64_t gpumpi_job_heap_bytes(const gpumpi_job* job);
 
 #ifdef __cplusplus
 }

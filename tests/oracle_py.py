@@ -95,3 +95,42 @@ def bucket_plan(orc, keys, key_bits, bucket_bits):
     orc.orc_bucket_plan(P, ctypes.c_int64(n), key_bits, bucket_bits, ptr(keys), ptr(kb1), ptr(sc),
                         ptr(sd), ptr(lo), ptr(hi))
     return kb1, sc, sd, lo, hi
+
+
+def job_dot(orc, seed, n, P):
+    """Dot-product job restated: lane-strided partial sums (64 lanes), butterfly over
+    lanes, then MPI_Allreduce SUM in rank order (out = in_0; out = in_r + out)."""
+    x = fill_normal(orc, seed, 0, n)
+    y = fill_normal(orc, seed, 1, n)
+    vals = []
+    for r in range(P):
+        lo, hi = n * r // P, n * (r + 1) // P
+        p = x[lo:hi] * y[lo:hi]
+        pad = (-p.size) % 64
+        p = np.concatenate([p, np.zeros(pad)]).reshape(-1, 64)
+        acc = np.zeros(64)
+        for row in p:
+            acc = acc + row
+        lanes = np.arange(64)
+        for o in (32, 16, 8, 4, 2, 1):
+            acc = acc + acc[lanes ^ o]
+        vals.append(acc[0])
+    out = vals[0]
+    for v in vals[1:]:
+        out = v + out
+    return float(out)
+
+
+def job_alltoall_checksum(orc, seed, m, P):
+    """Alltoall job restated: rank d receives block d of every rank's seeded payload; the
+    checksum is sum_i w_i (2i+1) mod 2^64 over 8-byte words, summed over ranks."""
+    send = [fill_bytes(orc, seed, s, P * m) for s in range(P)]
+    total = np.uint64(0)
+    for d in range(P):
+        rb = np.concatenate([send[s][d * m:(d + 1) * m] for s in range(P)])
+        pad = (-rb.size) % 8
+        w = np.concatenate([rb, np.zeros(pad, dtype=np.uint8)]).view(np.uint64)
+        k = (2 * np.arange(w.size, dtype=np.uint64) + np.uint64(1))
+        with np.errstate(over="ignore"):
+            total = total + np.sum(w * k, dtype=np.uint64)
+    return int(total)
