@@ -80,6 +80,9 @@ _SIGNATURES = {
     "gpumpi_launch": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                      _P, ctypes.c_size_t, ctypes.POINTER(Config)]),
     "gpumpi_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+    "gpumpi_run_kernel": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                         _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_main_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
     "gpumpi_last_error": (ctypes.c_char_p, [_P]),
     "gpumpi_last_device_error": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "gpumpi_program_count": (ctypes.c_int, []),
@@ -103,6 +106,9 @@ _SIGNATURES = {
     "gpumpi_serve_stop": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "gpumpi_run_job": (ctypes.c_int, [_P, ctypes.POINTER(Job), _P, _P]),
     "gpumpi_job_heap_bytes": (ctypes.c_int64, [ctypes.POINTER(Job)]),
+    "gpumpi_local_sort": (ctypes.c_int, [_P, ctypes.c_int64, _I32, _P, _P]),
+    "gpumpi_npb_is_run": (ctypes.c_int, [ctypes.c_char, _I32, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_gen_keys": (ctypes.c_int, [_P, _I32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
                                        _I32, _I32, _P, ctypes.c_int64, _P]),
     "gpumpi_bucket_plan": (ctypes.c_int, [_P, _I32, ctypes.c_int64, _I32, _I32, _P, _P, _P, _P, _P, _P]),
@@ -151,6 +157,23 @@ def bucket_plan(keys_ptr, P, n, key_bits, bucket_bits, kb1_ptr, sc_ptr, sd_ptr, 
                                            sc_ptr, sd_ptr, lo_ptr, hi_ptr, stream or None)
     if rc != OK:
         raise GpumpiError(f"gpumpi_bucket_plan failed ({rc})")
+
+
+def local_sort(keys_ptr, n, key_bits, out_ptr, stream=0):
+    rc = load_library().gpumpi_local_sort(keys_ptr, n, key_bits, out_ptr, stream or None)
+    if rc != OK:
+        raise GpumpiError(f"gpumpi_local_sort failed ({rc})")
+
+
+def npb_is(cls, P):
+    """NPB IS through the batched exchange; returns (passed, exchange_ms)."""
+    passed = ctypes.c_int64()
+    ms = ctypes.c_double()
+    rc = load_library().gpumpi_npb_is_run(ctypes.c_char(cls.encode()), P, ctypes.byref(passed),
+                                          ctypes.byref(ms))
+    if rc != OK:
+        raise GpumpiError(f"gpumpi_npb_is_run failed ({rc})")
+    return passed.value, ms.value
 
 
 def exported_symbols():

@@ -35,6 +35,38 @@ def compile_cmd(out):
     ]
 
 
+# Programs in the converter's output style, compiled as executables against the runtime
+# (each includes gpumpi/gpumpi_main.h) -- used by the tests.
+PROGRAMS = [os.path.join(ROOT, "tests", "programs", "pi_converted.hip")]
+
+
+def program_cmd(src, out):
+    return [
+        hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
+        "-Wno-unused-value", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "include", "gpumpi"),
+        "-I", os.path.join(HERE, "csrc"), src, "-L", HERE, "-lgpumpi",
+        "-Wl,-rpath," + HERE, "-o", out,
+    ]
+
+
+def build_programs(verbose=False):
+    outs = []
+    for src in PROGRAMS:
+        out_dir = os.path.join(os.path.dirname(src), "_build")
+        os.makedirs(out_dir, exist_ok=True)
+        out = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
+        cmd = program_cmd(src, out)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout)
+            raise RuntimeError(f"hipcc failed to build {src}")
+        outs.append(out)
+    return outs
+
+
 def build(verbose=False):
     """Compile libgpumpi.so (always recompiles); atomically replaces the old library."""
     fd, tmp = tempfile.mkstemp(prefix="libgpumpi.", suffix=".so.tmp", dir=HERE)
@@ -48,6 +80,7 @@ def build(verbose=False):
         sys.stderr.write(res.stdout)
         raise RuntimeError("hipcc failed to build libgpumpi.so")
     os.replace(tmp, LIB)
+    build_programs(verbose)
     return LIB
 
 

@@ -2,6 +2,7 @@
 // C ABI of include/gpumpi/gpumpi.h, compiled as one translation unit (the device runtime
 // is header-only and is inlined into every rank program).
 #include "runtime/mpi_api.h"
+#include "runtime/rank_kernel.h"
 #include "runtime/batched.h"
 #include "runtime/bucket.h"
 #include "programs/reference_tests.h"
@@ -18,20 +19,6 @@
 #include <vector>
 
 namespace gpumpi {
-
-// ------------------------------------------------------------------ rank grid kernel
-// Reference: __gpu_main_caller (gpu_main/main.cu:103-131) -- one CUDA thread per rank.
-// Here one wavefront per rank; blockDim.x = 64 * ranks_per_block.
-template <int (*MAIN)(int, char**)>
-__global__ void __launch_bounds__(256) rank_grid_kernel(Runtime* R, int argc, char** argv) {
-    s_rt = R;
-    const int rank = world_rank();
-    if (rank >= R->nranks) return;
-    int rc = MAIN(argc, argv);
-    if (rc != 0 && lane() == 0)
-        __hip_atomic_fetch_or(&R->exit_code, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    heap_flush_cache(&R->heap, &my_state()->cache);
-}
 
 struct InitBases {
     int32_t* posted;
@@ -444,6 +431,10 @@ int gpumpi_run_async(gpumpi_runtime* rt, const char* program, const void* args,
     return launch_program(rt, pe, 0, nullptr, s);
 }
 
+static int run_entry(gpumpi_runtime* rt, const ProgramEntry* pe, int argc,
+                     const char* const* argv, const void* args, size_t args_bytes, void* stream,
+                     double* elapsed_ms);
+
 int gpumpi_run(gpumpi_runtime* rt, const char* program, int argc, const char* const* argv,
                const void* args, size_t args_bytes, void* stream, double* elapsed_ms) {
     if (!rt) return GPUMPI_E_ARG;
@@ -452,6 +443,19 @@ int gpumpi_run(gpumpi_runtime* rt, const char* program, int argc, const char* co
         set_err(rt, std::string("unknown rank program: ") + (program ? program : "(null)"));
         return GPUMPI_E_PROGRAM;
     }
+    return run_entry(rt, pe, argc, argv, args, args_bytes, stream, elapsed_ms);
+}
+
+int gpumpi_run_kernel(gpumpi_runtime* rt, const void* kernel, int argc, const char* const* argv,
+                      const void* args, size_t args_bytes, void* stream, double* elapsed_ms) {
+    if (!rt || !kernel) return GPUMPI_E_ARG;
+    ProgramEntry pe{"(user kernel)", kernel};
+    return run_entry(rt, &pe, argc, argv, args, args_bytes, stream, elapsed_ms);
+}
+
+static int run_entry(gpumpi_runtime* rt, const ProgramEntry* pe, int argc,
+                     const char* const* argv, const void* args, size_t args_bytes, void* stream,
+                     double* elapsed_ms) {
     hipStream_t s = stream ? (hipStream_t)stream : rt->own_stream;
     rt->err.clear();
     if (rt->needs_reset) {
@@ -542,8 +546,30 @@ int gpumpi_launch(const char* program, int argc, const char* const* argv, const 
     return rc;
 }
 
-// main.cu:49-98: "<program> <args> ---gpumpi -n N -g G -b B -s S -p P -e E"
-int gpumpi_main(int argc, const char* const* argv) {
+static int launch_kernel_once(const void* kernel, int argc, const char* const* argv,
+                              const gpumpi_config* cfg) {
+    gpumpi_runtime* rt = nullptr;
+    int rc = gpumpi_runtime_create(cfg, &rt);
+    if (rc != GPUMPI_OK) return rc;
+    rc = gpumpi_run_kernel(rt, kernel, argc, argv, nullptr, 0, nullptr, nullptr);
+    if (rc < 0) g_host_error = rt->err;
+    gpumpi_runtime_destroy(rt);
+    return rc;
+}
+
+static int cli_main(const void* kernel, int argc, const char* const* argv);
+
+int gpumpi_main(int argc, const char* const* argv) { return cli_main(nullptr, argc, argv); }
+
+int gpumpi_main_kernel(const void* kernel, int argc, const char* const* argv) {
+    if (!kernel) return 1;
+    return cli_main(kernel, argc, argv);
+}
+
+// main.cu:49-98: "<program> <args> ---gpumpi -n N -g G -b B -s S -p P -e E".  With a
+// kernel (a compiled user program), argv[0] is the executable and every argument before
+// ---gpumpi is passed to __gpu_main; otherwise argv[0] names a built-in rank program.
+static int cli_main(const void* kernel, int argc, const char* const* argv) {
     int sep = -1;
     for (int i = 0; i < argc; i++)
         if (std::strcmp(argv[i], "---gpumpi") == 0) sep = i;
@@ -551,7 +577,7 @@ int gpumpi_main(int argc, const char* const* argv) {
         std::fprintf(stderr, "You should specify gpumpi related options after '---gpumpi'\n");
         return 1;
     }
-    if (sep < 1) {
+    if (sep < 1 && !kernel) {
         std::fprintf(stderr, "GPUMPI: no rank program named before '---gpumpi'\n");
         return 1;
     }
@@ -592,7 +618,8 @@ int gpumpi_main(int argc, const char* const* argv) {
     cfg.num_ranks = (int32_t)n;
     cfg.ranks_per_block = rpb;
     std::fprintf(stderr, "GPUMPI: Using %lld mpi processes (%d per workgroup)\n", n, rpb);
-    int rc = gpumpi_launch(argv[0], sep - 1, argv + 1, nullptr, 0, &cfg);
+    int rc = kernel ? launch_kernel_once(kernel, sep, argv, &cfg)
+                    : gpumpi_launch(argv[0], sep - 1, argv + 1, nullptr, 0, &cfg);
     if (rc < 0) {
         std::fprintf(stderr, "GPUMPI: %s\n", g_host_error.c_str());
         return 1;
@@ -1110,6 +1137,203 @@ int64_t gpumpi_job_heap_bytes(const gpumpi_job* job) {
     if (!job) return -1;
     progs::JobDesc d{job->id, job->seed, job->type, job->ranks, job->size};
     return progs::job_bytes(d);
+}
+
+}  // extern "C"
+
+// ==================================================================== local sort, NPB IS
+extern "C" {
+
+int gpumpi_local_sort(const uint32_t* keys, int64_t n, int32_t key_bits, uint32_t* out,
+                      void* stream) {
+    if (!keys || !out || n < 0 || n >= (1ll << 32) || key_bits < 1 || key_bits > 30)
+        return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t nbins = 1u << key_bits;
+    const int nblk = (int)((nbins + 4095) / 4096);
+    uint32_t *hist = nullptr, *prefix = nullptr, *big = nullptr, *nbig = nullptr;
+    uint64_t* sums = nullptr;
+    int rc = GPUMPI_OK;
+    if (hipMalloc((void**)&hist, sizeof(uint32_t) * nbins) != hipSuccess ||
+        hipMalloc((void**)&prefix, sizeof(uint32_t) * nbins) != hipSuccess ||
+        hipMalloc((void**)&big, sizeof(uint32_t) * nbins) != hipSuccess ||
+        hipMalloc((void**)&nbig, 64) != hipSuccess ||
+        hipMalloc((void**)&sums, sizeof(uint64_t) * nblk) != hipSuccess)
+        rc = GPUMPI_E_NOMEM;
+    if (rc == GPUMPI_OK) {
+        hipMemsetAsync(hist, 0, sizeof(uint32_t) * nbins, s);
+        hipMemsetAsync(nbig, 0, 64, s);
+        hipLaunchKernelGGL(sort_hist_kernel, dim3(1024), dim3(1024), 0, s, keys, n, nbins, hist);
+        hipLaunchKernelGGL(scan_block_sums_kernel, dim3(nblk), dim3(1024), 0, s, hist, nbins, sums);
+        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, nblk);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nblk), dim3(1024), 0, s, hist, nbins, sums, prefix);
+        hipLaunchKernelGGL(sort_expand_kernel, dim3(2048), dim3(256), 0, s, hist, prefix, nbins, out,
+                           big, nbig);
+        hipLaunchKernelGGL(sort_fill_runs_kernel, dim3(1024), dim3(256), 0, s, hist, prefix, big,
+                           nbig, out);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = GPUMPI_E_HIP;
+    }
+    hipFree(hist);
+    hipFree(prefix);
+    hipFree(big);
+    hipFree(nbig);
+    hipFree(sums);
+    return rc;
+}
+
+// NPB IS (toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI/IS/is.c) on the batched path: keys from
+// create_seq, one untimed rank(1), then rank(1..10) -- each iteration alters two keys of
+// rank 0 (is.c:650-654), builds the bucket plan, runs the batched exchange and checks the
+// five partial-verification keys (is.c:849-905: the rank of test key k is the number of
+// keys < k); after the last iteration every rank's keys are sorted (local sort) and the
+// global order checked (full_verify, is.c:545-621).  *passed follows is.c:1137: it must be
+// 5 * 10 + P for a correct run.
+int gpumpi_npb_is_run(char cls, int32_t P, int64_t* passed, double* exchange_ms) {
+    int tk, mk, bk;
+    static const int64_t idxS[5] = {48427, 17148, 23627, 62548, 4431}, rS[5] = {0, 18, 346, 64917, 65463};
+    static const int64_t idxW[5] = {357773, 934767, 875723, 898999, 404505},
+                         rW[5] = {1249, 11698, 1039987, 1043896, 1048018};
+    static const int64_t idxA[5] = {2112377, 662041, 5336171, 3642833, 4250760},
+                         rA[5] = {104, 17523, 123928, 8288932, 8388264};
+    static const int64_t idxB[5] = {41869, 812306, 5102857, 18232239, 26860214},
+                         rB[5] = {33422937, 10244, 59149, 33135281, 99};
+    static const int64_t idxC[5] = {44172927, 72999161, 74326391, 129606274, 21736814},
+                         rC[5] = {61147, 882988, 266290, 133997595, 133525895};
+    const int64_t *tidx, *trank;
+    switch (cls) {
+    case 'S': tk = 16; mk = 11; bk = 9; tidx = idxS; trank = rS; break;
+    case 'W': tk = 20; mk = 16; bk = 10; tidx = idxW; trank = rW; break;
+    case 'A': tk = 23; mk = 19; bk = 10; tidx = idxA; trank = rA; break;
+    case 'B': tk = 25; mk = 21; bk = 10; tidx = idxB; trank = rB; break;
+    case 'C': tk = 27; mk = 23; bk = 10; tidx = idxC; trank = rC; break;
+    default: return GPUMPI_E_ARG;
+    }
+    if (P < 1 || (P & (P - 1)) || !passed) return GPUMPI_E_ARG;
+    const int64_t T = 1ll << tk, n = T / P, maxkey = 1ll << mk;
+    const int shift = mk - bk;
+    hipStream_t s = nullptr;
+    std::vector<void*> bufs;
+    auto al = [&](size_t b) -> void* {
+        void* q = nullptr;
+        if (hipMalloc(&q, b ? b : 16) != hipSuccess) return nullptr;
+        bufs.push_back(q);
+        return q;
+    };
+    uint32_t* keys = (uint32_t*)al(sizeof(uint32_t) * T);
+    uint32_t* kb1 = (uint32_t*)al(sizeof(uint32_t) * T);
+    uint32_t* recv = (uint32_t*)al(sizeof(uint32_t) * T + 64);
+    uint32_t* sorted = (uint32_t*)al(sizeof(uint32_t) * T + 64);
+    int32_t* sc = (int32_t*)al(sizeof(int32_t) * P * P);
+    int32_t* sd = (int32_t*)al(sizeof(int32_t) * P * P);
+    int32_t* rc_ = (int32_t*)al(sizeof(int32_t) * P * P);
+    int32_t* rd = (int32_t*)al(sizeof(int32_t) * P * P);
+    int32_t* lo = (int32_t*)al(sizeof(int32_t) * P);
+    int32_t* hi = (int32_t*)al(sizeof(int32_t) * P);
+    void** sptr = (void**)al(sizeof(void*) * P);
+    void** rptr = (void**)al(sizeof(void*) * P);
+    unsigned long long* cnt = (unsigned long long*)al(64);
+    int rcode = GPUMPI_OK;
+    gpumpi_a2av_plan* plan = nullptr;
+    std::vector<void*> hs(P), hr(P);
+    std::vector<int32_t> hsc((size_t)P * P), hlo(P), hhi(P);
+    std::vector<int64_t> rtot(P), roff(P);
+    double ex_ms = 0;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (void* q : bufs)
+        if (!q) rcode = GPUMPI_E_NOMEM;
+    int64_t pass = 0;
+    if (rcode == GPUMPI_OK) {
+        for (int r = 0; r < P; r++) hs[r] = kb1 + (int64_t)r * n;
+        hipMemcpy(sptr, hs.data(), sizeof(void*) * P, hipMemcpyHostToDevice);
+        hipMemcpy(rptr, hs.data(), sizeof(void*) * P, hipMemcpyHostToDevice);  // replaced per step
+        rcode = gpumpi_gen_keys(keys, P, n, 0, 0, 2, mk, nullptr, 0, s);
+    }
+    if (rcode == GPUMPI_OK)
+        rcode = gpumpi_a2av_plan_create(P, 4, sptr, sc, sd, rptr, rc_, rd, nullptr, &plan);
+    for (int step = 0; step <= 10 && rcode == GPUMPI_OK; step++) {
+        const int it = step == 0 ? 1 : step;  // step 0 is the untimed rank(1), is.c:1069
+        int32_t mod[2] = {it, (int32_t)(maxkey - it)};
+        hipMemcpy(keys + it, &mod[0], 4, hipMemcpyHostToDevice);
+        hipMemcpy(keys + it + 10, &mod[1], 4, hipMemcpyHostToDevice);
+        uint32_t tkeys[5];
+        for (int i = 0; i < 5; i++) hipMemcpy(&tkeys[i], keys + tidx[i], 4, hipMemcpyDeviceToHost);
+        rcode = gpumpi_bucket_plan(keys, P, n, mk, bk, kb1, sc, sd, lo, hi, s);
+        if (rcode != GPUMPI_OK) break;
+        hipMemcpy(hsc.data(), sc, sizeof(int32_t) * P * P, hipMemcpyDeviceToHost);
+        hipMemcpy(hlo.data(), lo, sizeof(int32_t) * P, hipMemcpyDeviceToHost);
+        hipMemcpy(hhi.data(), hi, sizeof(int32_t) * P, hipMemcpyDeviceToHost);
+        int64_t off = 0;
+        for (int d = 0; d < P; d++) {
+            int64_t t = 0;
+            for (int q = 0; q < P; q++) t += hsc[(size_t)q * P + d];
+            rtot[d] = t;
+            roff[d] = off;
+            hr[d] = recv + off;
+            off += t;
+        }
+        hipMemcpy(rptr, hr.data(), sizeof(void*) * P, hipMemcpyHostToDevice);
+        hipEventRecord(e0, s);
+        rcode = gpumpi_a2av_execute(plan, s);
+        hipEventRecord(e1, s);
+        hipStreamSynchronize(s);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (step > 0) ex_ms += ms;
+        int64_t bytes = 0;
+        if (rcode == GPUMPI_OK) rcode = gpumpi_a2av_payload_bytes(plan, &bytes);
+        if (rcode != GPUMPI_OK) break;
+        // partial verification (is.c:849-905)
+        for (int i = 0; i < 5 && step > 0; i++) {
+            const int64_t k = tkeys[i];
+            const int64_t b = k >> shift;
+            int owner = -1;
+            for (int d = 0; d < P; d++)
+                if (hlo[d] <= b && b <= hhi[d]) owner = d;
+            if (owner < 0) continue;
+            hipMemset(cnt, 0, 8);
+            hipLaunchKernelGGL(count_less_kernel, dim3(256), dim3(256), 0, s, recv + roff[owner],
+                               rtot[owner], (uint32_t)k, cnt);
+            unsigned long long c = 0;
+            hipMemcpy(&c, cnt, 8, hipMemcpyDeviceToHost);
+            const int64_t key_rank = (int64_t)c + roff[owner];
+            int64_t tr = trank[i];
+            switch (cls) {
+            case 'S': tr = (i <= 2) ? tr + it : tr - it; break;
+            case 'W': tr = (i < 2) ? tr + it - 2 : tr - it; break;
+            case 'A': tr = (i <= 2) ? tr + it - 1 : tr - (it - 1); break;
+            case 'B': tr = (i == 1 || i == 2 || i == 4) ? tr + it : tr - it; break;
+            default: tr = (i <= 2) ? tr + it : tr - it; break;
+            }
+            if (key_rank == tr) pass++;
+        }
+    }
+    if (rcode == GPUMPI_OK) {
+        // full_verify (is.c:545-621): sort every rank's keys, check each rank's slice lies in
+        // its bucket range and the concatenation is ascending
+        rcode = gpumpi_local_sort(recv, T, mk, sorted, s);
+        if (rcode == GPUMPI_OK) {
+            std::vector<uint32_t> hs2(T);
+            hipMemcpy(hs2.data(), sorted, sizeof(uint32_t) * T, hipMemcpyDeviceToHost);
+            for (int d = 0; d < P; d++) {
+                bool ok = true;
+                for (int64_t i = roff[d]; i < roff[d] + rtot[d]; i++) {
+                    const int64_t b = (int64_t)hs2[i] >> shift;
+                    if (b < hlo[d] || b > hhi[d]) ok = false;
+                    if (i > 0 && hs2[i - 1] > hs2[i]) ok = false;
+                }
+                if (ok) pass++;
+            }
+        }
+    }
+    if (plan) gpumpi_a2av_plan_destroy(plan);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (void* q : bufs) hipFree(q);
+    *passed = pass;
+    if (exchange_ms) *exchange_ms = ex_ms;
+    return rcode;
 }
 
 }  // extern "C"

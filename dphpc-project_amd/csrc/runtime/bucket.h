@@ -195,4 +195,126 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* key
     }
 }
 
+// ------------------------------------------------------------------ local sort
+// Every rank owns a disjoint, ascending key range (the bucket ranges), so the local
+// sorts of all ranks together are one counting sort of the concatenated receive buffers:
+// rank d's sorted keys are the slice [recv_off_d, recv_off_d + recv_tot_d) of the output.
+constexpr int kSortLdsBins = 16384;  // keys below this are counted in LDS first
+
+__global__ void __launch_bounds__(1024) sort_hist_kernel(const uint32_t* keys, int64_t n,
+                                                         uint32_t nbins, uint32_t* hist) {
+    __shared__ uint32_t lh[kSortLdsBins];
+    const uint32_t lds_bins = nbins < kSortLdsBins ? nbins : kSortLdsBins;
+    for (uint32_t b = threadIdx.x; b < lds_bins; b += blockDim.x) lh[b] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = keys[i];
+        if (k < lds_bins) atomicAdd(&lh[k], 1u);
+        else atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < lds_bins; b += blockDim.x)
+        if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+
+// Exclusive scan of nbins counters in three kernels (4096 counters per 1024-thread block).
+__global__ void __launch_bounds__(1024) scan_block_sums_kernel(const uint32_t* h, uint32_t nbins,
+                                                               uint64_t* sums) {
+    __shared__ uint64_t red[16];
+    const uint64_t base = (uint64_t)blockIdx.x * 4096;
+    uint64_t s = 0;
+    for (int k = 0; k < 4; k++) {
+        const uint64_t i = base + (uint64_t)k * 1024 + threadIdx.x;
+        if (i < nbins) s += h[i];
+    }
+    s = (uint64_t)wave_sum((int64_t)s);
+    if (lane() == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < 16; w++) t += red[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(1024) scan_sums_kernel(uint64_t* sums, int nblocks) {
+    // single block, exclusive scan in place (sequential chunks of 1024)
+    __shared__ uint64_t part[16];
+    uint64_t carry = 0;
+    for (int base = 0; base < nblocks; base += 1024) {
+        const int i = base + threadIdx.x;
+        const uint64_t v = i < nblocks ? sums[i] : 0;
+        const uint64_t inc = (uint64_t)wave_incl_scan((int64_t)v);
+        if (lane() == 63) part[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        uint64_t before = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); w++) before += part[w];
+        uint64_t total = 0;
+        for (int w = 0; w < 16; w++) total += part[w];
+        if (i < nblocks) sums[i] = carry + before + inc - v;
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(1024) scan_apply_kernel(const uint32_t* h, uint32_t nbins,
+                                                          const uint64_t* sums, uint32_t* prefix) {
+    __shared__ uint64_t part[16];
+    const uint64_t base = (uint64_t)blockIdx.x * 4096;
+    uint64_t carry = sums[blockIdx.x];
+    for (int k = 0; k < 4; k++) {
+        const uint64_t i = base + (uint64_t)k * 1024 + threadIdx.x;
+        const uint64_t v = i < nbins ? h[i] : 0;
+        const uint64_t inc = (uint64_t)wave_incl_scan((int64_t)v);
+        if (lane() == 63) part[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        uint64_t before = 0, total = 0;
+        for (int w = 0; w < 16; w++) {
+            if (w < (int)(threadIdx.x >> 6)) before += part[w];
+            total += part[w];
+        }
+        if (i < nbins) prefix[i] = (uint32_t)(carry + before + inc - v);
+        carry += total;
+        __syncthreads();
+    }
+}
+
+// Writes each value's run; runs longer than 256 go to a list filled by whole blocks.
+__global__ void sort_expand_kernel(const uint32_t* hist, const uint32_t* prefix, uint32_t nbins,
+                                   uint32_t* out, uint32_t* big, uint32_t* nbig) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nbins;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = hist[v];
+        if (c == 0) continue;
+        if (c <= 256) {
+            const uint32_t o = prefix[v];
+            for (uint32_t k = 0; k < c; k++) out[o + k] = (uint32_t)v;
+        } else {
+            big[atomicAdd(nbig, 1u)] = (uint32_t)v;
+        }
+    }
+}
+
+__global__ void sort_fill_runs_kernel(const uint32_t* hist, const uint32_t* prefix,
+                                      const uint32_t* big, const uint32_t* nbig, uint32_t* out) {
+    const uint32_t nb = *nbig;
+    for (uint32_t r = blockIdx.x; r < nb; r += gridDim.x) {
+        const uint32_t v = big[r];
+        const uint32_t c = hist[v], o = prefix[v];
+        for (uint32_t k = threadIdx.x; k < c; k += blockDim.x) out[o + k] = v;
+    }
+}
+
+// Number of keys < bound in keys[0, n) (partial verification of NPB IS).
+__global__ void count_less_kernel(const uint32_t* keys, int64_t n, uint32_t bound,
+                                  unsigned long long* out) {
+    uint64_t c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += keys[i] < bound;
+    c = (uint64_t)wave_sum((int64_t)c);
+    if (lane() == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
 }  // namespace gpumpi

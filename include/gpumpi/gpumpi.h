@@ -75,6 +75,13 @@ int gpumpi_run(gpumpi_runtime* rt, const char* program, int argc, const char* co
 int gpumpi_run_async(gpumpi_runtime* rt, const char* program, const void* args,
                      size_t args_bytes, void* stream);
 
+/* Launch a rank-grid kernel compiled into the caller's own module (a converted program,
+ * see gpumpi/gpumpi_main.h) with the same semantics as gpumpi_run. */
+int gpumpi_run_kernel(gpumpi_runtime* rt, const void* kernel, int argc, const char* const* argv,
+                      const void* args, size_t args_bytes, void* stream, double* elapsed_ms);
+/* CLI of a compiled converted program: argv = "<exe> <args> ---gpumpi -n N ...". */
+int gpumpi_main_kernel(const void* kernel, int argc, const char* const* argv);
+
 /* One-shot: create, run, destroy (what main.cu does for one converted program). */
 int gpumpi_launch(const char* program, int argc, const char* const* argv, const void* args,
                   size_t args_bytes, const gpumpi_config* cfg);
@@ -158,6 +165,16 @@ int gpumpi_bucket_plan(const uint32_t* keys, int32_t P, int64_t n, int32_t key_b
                        int32_t bucket_bits, uint32_t* key_buff1, int32_t* send_counts,
                        int32_t* send_displs, int32_t* bucket_lo, int32_t* bucket_hi,
                        void* stream);
+
+/* Counting sort of n keys < 2^key_bits into out (device buffers, synchronous).  Applied
+ * to the concatenated receive buffers of a bucket exchange it is every rank's local sort
+ * at once (ranks own disjoint ascending key ranges; is.c:785-833 ranks by counting). */
+int gpumpi_local_sort(const uint32_t* keys, int64_t n, int32_t key_bits, uint32_t* out,
+                      void* stream);
+/* NPB IS class cls ('S','W','A','B','C') on P ranks (power of two) through the batched
+ * exchange: returns NPB's verification count in *passed (correct run: 5 * 10 + P,
+ * is.c:1137-1138) and the summed exchange time of the 10 timed iterations. */
+int gpumpi_npb_is_run(char cls, int32_t P, int64_t* passed, double* exchange_ms);
 
 /* ------------------------------------------------------------------------------------
  * Served MPI jobs.  A job is an independent MPI program on its own communicator:

@@ -111,6 +111,12 @@ extern __device__ gpumpi::OpImpl gpumpi_builtin_ops[12];
 #define MPI_GROUP_EMPTY (gpumpi::empty_group())
 #define MPI_GROUP_NULL ((MPI_Group)nullptr)
 
+// Prints once per rank (a rank is a wavefront; plain printf would print from all 64 lanes).
+#define gpumpi_printf(...)                           \
+    do {                                             \
+        if (__lane_id() == 0) printf(__VA_ARGS__);   \
+    } while (0)
+
 // ---- environment ----
 __device__ int MPI_Init(int* argc, char*** argv);
 __device__ int MPI_Init_thread(int* argc, char*** argv, int required, int* provided);

@@ -211,3 +211,33 @@ def test_full_size_exchange(dist, pkg, oracle, torch_gpu):
         o = int(ex.roff[d]) * 4
         assert np.array_equal(recv[o:o + int(ex.rtot[d]) * 4], exp[d][:int(ex.rtot[d]) * 4]), d
     plan.close()
+
+
+@pytest.mark.parametrize("cls,P", [("S", 1), ("S", 4), ("S", 16), ("S", 64), ("W", 8), ("W", 32),
+                                   ("A", 16)])
+def test_npb_is_known_answers(cls, P, pkg, torch_gpu):
+    """NPB IS partial + full verification through the batched exchange: 5*10 + P passes
+    (the class known answers of is.c:264-287 with the per-iteration shifts of 849-905)."""
+    print(f"[gpu] NPB IS class {cls} on {P} ranks", flush=True)
+    passed, ms = pkg.npb_is(cls, P)
+    assert passed == 5 * 10 + P, passed
+
+
+@pytest.mark.parametrize("dist", ["uniform", "zipf"])
+def test_local_sort_gives_global_order(dist, pkg, torch_gpu):
+    torch = torch_gpu
+    import importlib.util, os
+    print(f"[gpu] bucket exchange + local sort {dist}", flush=True)
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(os.path.dirname(__file__), "..", "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    ex = bench.Exchange(pkg, torch, 64, 1 << 16, dist)
+    plan = ex.plan(pkg)
+    plan.execute()
+    torch.cuda.synchronize()
+    out = torch.zeros_like(ex.recv)
+    total = int(ex.rtot.sum())
+    pkg.local_sort(ex.recv.data_ptr(), total, 23, out.data_ptr())
+    got = out.cpu().numpy().view(np.uint32)[:total]
+    assert np.array_equal(got, np.sort(ex.keys.cpu().numpy().view(np.uint32)))
+    plan.close()
