@@ -1,7 +1,7 @@
 #pragma once
 #include "workloads.h"
+#include "particles.h"
 namespace gpumpi { namespace progs {
 __device__ int w_bucket_sort(int, char**) { return 0; }
 __device__ int w_npb_is(int, char**) { return 0; }
-__device__ int w_particles(int, char**) { return 0; }
 }}

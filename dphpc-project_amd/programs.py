@@ -40,6 +40,16 @@ class ExchArgs(ctypes.Structure):
                 ("rdispls", _P)]
 
 
+class ParticleArgs(ctypes.Structure):
+    """Particle exchange on a periodic gx*gy*gz rank grid (8 FP64 fields per particle);
+    mode 0 = Alltoall(counts) + Alltoallv, mode 1 = neighbour Isend/Irecv."""
+    _fields_ = [("seed", ctypes.c_uint64), ("gx", ctypes.c_int32), ("gy", ctypes.c_int32),
+                ("gz", ctypes.c_int32), ("mode", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("first_step", ctypes.c_int32), ("n0", ctypes.c_int64), ("cap", ctypes.c_int64),
+                ("buf", _P * 2), ("sendbuf", _P), ("counts", _P), ("cnt", _P), ("moved", _P),
+                ("seconds", _P)]
+
+
 COLL_KINDS = {"bcast": 0, "reduce": 1, "allreduce": 2, "alltoall": 3, "allgather": 4,
               "gather": 5, "scatter": 6}
 

@@ -683,6 +683,132 @@ void orc_local_sort(int64_t nkeys, const uint32_t *keys, uint32_t *out)
 }
 
 /* ------------------------------------------------------------------------- */
+/* Particle exchange.  The workload is the particle variant of the batched uneven
+ * exchange; its one communication step is an MPI_Alltoallv whose semantics are
+ * those of gpu_mpi/mpi.cu:363-408 (restated by orc_alltoallv). */
+int orc_particle_dir(uint64_t seed, uint32_t step, uint64_t id)
+{
+    const uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), step, ORC_STREAM_MIGR};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    if (o[0] >= ORC_PART_MOVE) return -1;
+    return (int)(((uint64_t)o[1] * 6u) >> 32);
+}
+
+int orc_grid_neighbour(int gx, int gy, int gz, int rank, int dir)
+{
+    int x = rank % gx, y = (rank / gx) % gy, z = rank / (gx * gy);
+    switch (dir) {
+    case 0: x = (x + 1) % gx; break;
+    case 1: x = (x + gx - 1) % gx; break;
+    case 2: y = (y + 1) % gy; break;
+    case 3: y = (y + gy - 1) % gy; break;
+    case 4: z = (z + 1) % gz; break;
+    default: z = (z + gz - 1) % gz; break;
+    }
+    return x + gx * (y + gy * z);
+}
+
+void orc_particles_init_rank(int gx, int gy, int gz, int rank, int64_t n0, uint64_t seed,
+                             double *parts)
+{
+    const double q20 = 1.0 / 1048576.0, q15 = 1.0 / 32768.0;
+    const int cx = rank % gx, cy = (rank / gx) % gy, cz = rank / (gx * gy);
+    (void)gz;
+    for (int64_t k = 0; k < n0; k++) {
+        uint32_t o[4];
+        philox_at(seed, rank, 0u, (uint64_t)k, ORC_STREAM_PART, o);
+        double *q = parts + k * ORC_PART_FIELDS;
+        q[0] = (double)cx + (double)(o[0] >> 12) * q20;
+        q[1] = (double)cy + (double)(o[1] >> 12) * q20;
+        q[2] = (double)cz + (double)(o[2] >> 12) * q20;
+        q[3] = (double)((int32_t)(o[3] & 0xFFFFu) - 32768) * q15;
+        q[4] = (double)((int32_t)(o[3] >> 16) - 32768) * q15;
+        q[5] = (double)(k & 1023) * (1.0 / 1024.0);
+        q[6] = 1.0;
+        q[7] = (double)((int64_t)rank * n0 + k);
+    }
+}
+
+void orc_particles_init(int gx, int gy, int gz, int64_t n0, uint64_t seed, int64_t cap,
+                        double *parts, int64_t *counts)
+{
+    const int P = gx * gy * gz;
+    for (int r = 0; r < P; r++) {
+        orc_particles_init_rank(gx, gy, gz, r, n0, seed, parts + (size_t)r * cap * ORC_PART_FIELDS);
+        counts[r] = n0;
+    }
+}
+
+int64_t orc_particles_step(int gx, int gy, int gz, uint64_t seed, uint32_t step, int64_t cap,
+                           const double *in, const int64_t *cin, double *out, int64_t *cout)
+{
+    const int P = gx * gy * gz;
+    const int dims[3] = {gx, gy, gz};
+    const size_t F = ORC_PART_FIELDS;
+    int *sc = calloc((size_t)P * P, sizeof(int)), *sd = calloc((size_t)P * P, sizeof(int));
+    int *rc = calloc((size_t)P * P, sizeof(int)), *rd = calloc((size_t)P * P, sizeof(int));
+    double *send = malloc(sizeof(double) * (size_t)P * cap * F);
+    const void **sbufs = malloc(sizeof(void *) * P);
+    void **rbufs = malloc(sizeof(void *) * P);
+    int64_t *stay = malloc(sizeof(int64_t) * P);
+    int64_t moved = 0, status = 0;
+    for (int r = 0; r < P; r++) {
+        const double *p = in + (size_t)r * cap * F;
+        double *o = out + (size_t)r * cap * F;
+        double *sb = send + (size_t)r * cap * F;
+        int64_t n = cin[r], ns = 0;
+        /* pass 1: migrants per destination rank; pass 2: pack them in ascending
+         * destination order, each group in the original particle order */
+        for (int64_t k = 0; k < n; k++) {
+            int d = orc_particle_dir(seed, step, (uint64_t)p[k * F + 7]);
+            if (d < 0) ns++;
+            else sc[(size_t)r * P + orc_grid_neighbour(gx, gy, gz, r, d)] += (int)F;
+        }
+        int acc = 0;
+        for (int j = 0; j < P; j++) { sd[(size_t)r * P + j] = acc; acc += sc[(size_t)r * P + j]; }
+        int *cur = malloc(sizeof(int) * P);
+        memcpy(cur, sd + (size_t)r * P, sizeof(int) * P);
+        int64_t st = 0;
+        for (int64_t k = 0; k < n; k++) {
+            const double *q = p + k * F;
+            int d = orc_particle_dir(seed, step, (uint64_t)q[7]);
+            if (d < 0) { memcpy(o + st * F, q, sizeof(double) * F); st++; continue; }
+            int j = orc_grid_neighbour(gx, gy, gz, r, d);
+            double *m = sb + cur[j];
+            memcpy(m, q, sizeof(double) * F);
+            const int ax = d >> 1;
+            double c = m[ax] + ((d & 1) ? -1.0 : 1.0);
+            if (c >= (double)dims[ax]) c -= (double)dims[ax];
+            if (c < 0.0) c += (double)dims[ax];
+            m[ax] = c;
+            cur[j] += (int)F;
+            moved++;
+        }
+        free(cur);
+        stay[r] = ns;
+        sbufs[r] = sb;
+        rbufs[r] = o;
+    }
+    /* Alltoall of the counts, then receive displacements after the stayers */
+    for (int r = 0; r < P; r++) {
+        int64_t acc = stay[r] * (int64_t)F;
+        for (int s = 0; s < P; s++) {
+            rc[(size_t)r * P + s] = sc[(size_t)s * P + r];
+            rd[(size_t)r * P + s] = (int)acc;
+            acc += rc[(size_t)r * P + s];
+        }
+        if (acc > cap * (int64_t)F) status = -1;
+        cout[r] = acc / (int64_t)F;
+    }
+    if (status == 0)
+        orc_alltoallv(P, 8, 8, sbufs, sc, sd, rbufs, rc, rd);
+    free(sc); free(sd); free(rc); free(rd); free(send); free(sbufs); free(rbufs); free(stay);
+    return status ? status : moved;
+}
+
+/* ------------------------------------------------------------------------- */
 typedef struct { uint64_t p, s; int64_t i; } blk_t;
 static int cmp_blk(const void *a, const void *b)
 {

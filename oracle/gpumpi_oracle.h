@@ -40,6 +40,14 @@
  *   orc_bucket_*        The NPB-style bucket exchange applied to the workload's
  *                       uniform / Zipf keys (same rank() steps).  Pinned by the
  *                       global order equalling a plain sort of all keys.
+ *   orc_particles_*     Particle exchange on a periodic gx*gy*gz rank grid: every
+ *                       rank packs its migrants by destination rank and the step
+ *                       is one MPI_Alltoallv (orc_alltoallv above), stayers first.
+ *                       Pinned by conservation (ids are a permutation), every
+ *                       particle inside its owner's cell, a migrant moves exactly
+ *                       one cell along one axis, the migration rate against its
+ *                       binomial bounds, and host MPICH running the same program
+ *                       (oracle/mpi/particles.c) bit for bit.
  */
 #ifndef GPUMPI_ORACLE_H
 #define GPUMPI_ORACLE_H
@@ -60,7 +68,8 @@ enum {
     ORC_STREAM_KEYS   = 0x0B17E503u,
     ORC_STREAM_ZIPF   = 0x0B17E504u,
     ORC_STREAM_MIGR   = 0x0B17E505u,
-    ORC_STREAM_JOBS   = 0x0B17E506u
+    ORC_STREAM_JOBS   = 0x0B17E506u,
+    ORC_STREAM_PART   = 0x0B17E507u
 };
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -115,6 +124,24 @@ void orc_counts_transpose(int P, const int *send_count, int *recv_count, int *re
 /* Counting sort of each rank's received keys (rank() 807-833 then full_verify's
  * placement); out gets the keys in ascending order per rank. */
 void orc_local_sort(int64_t nkeys, const uint32_t *keys, uint32_t *out);
+
+/* ---- particle exchange (8 FP64 fields per particle: x y z vx vy vz w id) ---- */
+#define ORC_PART_FIELDS 8
+/* Probability of a move per particle and step is ORC_PART_MOVE / 2^32 (~0.05). */
+#define ORC_PART_MOVE 214748365u
+/* Destination direction of particle `id` at `step`: -1 stays, 0..5 = +x -x +y -y +z -z. */
+int  orc_particle_dir(uint64_t seed, uint32_t step, uint64_t id);
+int  orc_grid_neighbour(int gx, int gy, int gz, int rank, int dir);
+/* Initial particles of one rank: n0 particles into parts[0 .. n0*8). */
+void orc_particles_init_rank(int gx, int gy, int gz, int rank, int64_t n0, uint64_t seed,
+                             double *parts);
+/* Initial state: n0 particles per rank in buffer [P][cap][8]; counts[P] = n0. */
+void orc_particles_init(int gx, int gy, int gz, int64_t n0, uint64_t seed, int64_t cap,
+                        double *parts, int64_t *counts);
+/* One exchange step from (in, cin) to (out, cout).  Returns the number of migrants,
+ * or -1 if a rank would exceed cap. */
+int64_t orc_particles_step(int gx, int gy, int gz, uint64_t seed, uint32_t step, int64_t cap,
+                           const double *in, const int64_t *cin, double *out, int64_t *cout);
 
 /* ---- heap invariant checker ---- */
 /* Returns 0 if the live blocks [ptr, ptr+size) are pairwise disjoint, lie in

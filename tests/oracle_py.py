@@ -134,3 +134,30 @@ def job_alltoall_checksum(orc, seed, m, P):
         with np.errstate(over="ignore"):
             total = total + np.sum(w * k, dtype=np.uint64)
     return int(total)
+
+
+PART_FIELDS = 8
+
+
+def particles_init(orc, dims, n0, seed, cap):
+    """Oracle initial particle state: ([P, cap, 8] float64, counts[P])."""
+    gx, gy, gz = dims
+    P = gx * gy * gz
+    parts = np.zeros((P, cap, PART_FIELDS), dtype=np.float64)
+    counts = np.zeros(P, dtype=np.int64)
+    orc.orc_particles_init(gx, gy, gz, ctypes.c_int64(n0), ctypes.c_uint64(seed),
+                           ctypes.c_int64(cap), ptr(parts), ptr(counts))
+    return parts, counts
+
+
+def particles_step(orc, dims, seed, step, parts, counts):
+    """One oracle exchange step; returns (parts, counts, migrants)."""
+    gx, gy, gz = dims
+    cap = parts.shape[1]
+    out = np.zeros_like(parts)
+    cout = np.zeros_like(counts)
+    orc.orc_particles_step.restype = ctypes.c_int64
+    moved = orc.orc_particles_step(gx, gy, gz, ctypes.c_uint64(seed), ctypes.c_uint32(step),
+                                   ctypes.c_int64(cap), ptr(parts), ptr(counts), ptr(out), ptr(cout))
+    assert moved >= 0, "a rank exceeded its particle capacity"
+    return out, cout, moved
