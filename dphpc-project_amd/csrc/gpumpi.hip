@@ -136,7 +136,6 @@ static const ProgramEntry kPrograms[] = {
     GPUMPI_PROGRAM("churn", progs::w_churn),
     GPUMPI_PROGRAM("exchange", progs::w_exchange),
     GPUMPI_PROGRAM("bucket_sort", progs::w_bucket_sort),
-    GPUMPI_PROGRAM("npb_is", progs::w_npb_is),
     GPUMPI_PROGRAM("particles", progs::w_particles),
     GPUMPI_PROGRAM("job", progs::w_job),
     GPUMPI_PROGRAM("serve", progs::w_serve),
@@ -1189,25 +1188,11 @@ int gpumpi_local_sort(const uint32_t* keys, int64_t n, int32_t key_bits, uint32_
 // global order checked (full_verify, is.c:545-621).  *passed follows is.c:1137: it must be
 // 5 * 10 + P for a correct run.
 int gpumpi_npb_is_run(char cls, int32_t P, int64_t* passed, double* exchange_ms) {
-    int tk, mk, bk;
-    static const int64_t idxS[5] = {48427, 17148, 23627, 62548, 4431}, rS[5] = {0, 18, 346, 64917, 65463};
-    static const int64_t idxW[5] = {357773, 934767, 875723, 898999, 404505},
-                         rW[5] = {1249, 11698, 1039987, 1043896, 1048018};
-    static const int64_t idxA[5] = {2112377, 662041, 5336171, 3642833, 4250760},
-                         rA[5] = {104, 17523, 123928, 8288932, 8388264};
-    static const int64_t idxB[5] = {41869, 812306, 5102857, 18232239, 26860214},
-                         rB[5] = {33422937, 10244, 59149, 33135281, 99};
-    static const int64_t idxC[5] = {44172927, 72999161, 74326391, 129606274, 21736814},
-                         rC[5] = {61147, 882988, 266290, 133997595, 133525895};
-    const int64_t *tidx, *trank;
-    switch (cls) {
-    case 'S': tk = 16; mk = 11; bk = 9; tidx = idxS; trank = rS; break;
-    case 'W': tk = 20; mk = 16; bk = 10; tidx = idxW; trank = rW; break;
-    case 'A': tk = 23; mk = 19; bk = 10; tidx = idxA; trank = rA; break;
-    case 'B': tk = 25; mk = 21; bk = 10; tidx = idxB; trank = rB; break;
-    case 'C': tk = 27; mk = 23; bk = 10; tidx = idxC; trank = rC; break;
-    default: return GPUMPI_E_ARG;
-    }
+    NpbClassInfo ci;
+    if (!npb_class_info(cls, ci)) return GPUMPI_E_ARG;
+    const int tk = ci.total_keys_log2, mk = ci.max_key_log2, bk = ci.num_buckets_log2;
+    const int64_t* tidx = ci.test_index;
+    const int64_t* trank = ci.test_rank;
     if (P < 1 || (P & (P - 1)) || !passed) return GPUMPI_E_ARG;
     const int64_t T = 1ll << tk, n = T / P, maxkey = 1ll << mk;
     const int shift = mk - bk;
@@ -1298,14 +1283,7 @@ int gpumpi_npb_is_run(char cls, int32_t P, int64_t* passed, double* exchange_ms)
             unsigned long long c = 0;
             hipMemcpy(&c, cnt, 8, hipMemcpyDeviceToHost);
             const int64_t key_rank = (int64_t)c + roff[owner];
-            int64_t tr = trank[i];
-            switch (cls) {
-            case 'S': tr = (i <= 2) ? tr + it : tr - it; break;
-            case 'W': tr = (i < 2) ? tr + it - 2 : tr - it; break;
-            case 'A': tr = (i <= 2) ? tr + it - 1 : tr - (it - 1); break;
-            case 'B': tr = (i == 1 || i == 2 || i == 4) ? tr + it : tr - it; break;
-            default: tr = (i <= 2) ? tr + it : tr - it; break;
-            }
+            const int64_t tr = npb_expected_rank(cls, i, it, trank[i]);
             if (key_rank == tr) pass++;
         }
     }

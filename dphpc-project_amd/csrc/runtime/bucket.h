@@ -42,6 +42,100 @@ __host__ __device__ __forceinline__ uint64_t npb_rank_seed(int kn, int np, int64
     return (kNpbSeed * lcg_pow(an, (uint64_t)kn)) & kMask46;
 }
 
+// NPB IS class parameters and partial-verification tables (is.c:57-104, 264-287).
+struct NpbClassInfo {
+    int total_keys_log2, max_key_log2, num_buckets_log2;
+    int64_t test_index[5], test_rank[5];
+};
+
+__host__ __device__ inline bool npb_class_info(char cls, NpbClassInfo& c) {
+    int64_t ti[5], tr[5];
+    switch (cls) {
+    case 'S': {
+        c.total_keys_log2 = 16; c.max_key_log2 = 11; c.num_buckets_log2 = 9;
+        const int64_t a[5] = {48427, 17148, 23627, 62548, 4431}, b[5] = {0, 18, 346, 64917, 65463};
+        for (int i = 0; i < 5; i++) { ti[i] = a[i]; tr[i] = b[i]; }
+        break;
+    }
+    case 'W': {
+        c.total_keys_log2 = 20; c.max_key_log2 = 16; c.num_buckets_log2 = 10;
+        const int64_t a[5] = {357773, 934767, 875723, 898999, 404505},
+                      b[5] = {1249, 11698, 1039987, 1043896, 1048018};
+        for (int i = 0; i < 5; i++) { ti[i] = a[i]; tr[i] = b[i]; }
+        break;
+    }
+    case 'A': {
+        c.total_keys_log2 = 23; c.max_key_log2 = 19; c.num_buckets_log2 = 10;
+        const int64_t a[5] = {2112377, 662041, 5336171, 3642833, 4250760},
+                      b[5] = {104, 17523, 123928, 8288932, 8388264};
+        for (int i = 0; i < 5; i++) { ti[i] = a[i]; tr[i] = b[i]; }
+        break;
+    }
+    case 'B': {
+        c.total_keys_log2 = 25; c.max_key_log2 = 21; c.num_buckets_log2 = 10;
+        const int64_t a[5] = {41869, 812306, 5102857, 18232239, 26860214},
+                      b[5] = {33422937, 10244, 59149, 33135281, 99};
+        for (int i = 0; i < 5; i++) { ti[i] = a[i]; tr[i] = b[i]; }
+        break;
+    }
+    case 'C': {
+        c.total_keys_log2 = 27; c.max_key_log2 = 23; c.num_buckets_log2 = 10;
+        const int64_t a[5] = {44172927, 72999161, 74326391, 129606274, 21736814},
+                      b[5] = {61147, 882988, 266290, 133997595, 133525895};
+        for (int i = 0; i < 5; i++) { ti[i] = a[i]; tr[i] = b[i]; }
+        break;
+    }
+    default:
+        return false;
+    }
+    for (int i = 0; i < 5; i++) { c.test_index[i] = ti[i]; c.test_rank[i] = tr[i]; }
+    return true;
+}
+
+// Expected rank of test key i at `iteration` (is.c:849-905).
+__host__ __device__ inline int64_t npb_expected_rank(char cls, int i, int iteration, int64_t r) {
+    switch (cls) {
+    case 'S': return i <= 2 ? r + iteration : r - iteration;
+    case 'W': return i < 2 ? r + iteration - 2 : r - iteration;
+    case 'A': return i <= 2 ? r + iteration - 1 : r - (iteration - 1);
+    case 'B': return (i == 1 || i == 2 || i == 4) ? r + iteration : r - iteration;
+    default: return i <= 2 ? r + iteration : r - iteration;  // 'C'
+    }
+}
+
+// Key i of rank r for the uniform (dist 0) and Zipf (dist 1) workloads.
+__host__ __device__ __forceinline__ uint32_t gen_key(int dist, uint64_t seed, int r, uint32_t step,
+                                                     int64_t i, int key_bits, const double* cdf,
+                                                     int64_t ncdf) {
+    if (dist == 0) {
+        u32x4v v = rank_stream(seed, r, step, (uint64_t)(i >> 2), kStreamKeys);
+        return pick(v, (int)(i & 3)) >> (32 - key_bits);
+    }
+    u32x4v v = rank_stream(seed, r, step, (uint64_t)(i >> 1), kStreamZipf);
+    const int h = 2 * (int)(i & 1);
+    const double u = unit53(pick(v, h), pick(v, h + 1));
+    int64_t lo = 0, hi = ncdf - 1;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    return (uint32_t)lo;
+}
+
+// NPB IS create_seq key i of rank r: (x1+x2+x3+x4) >> (48 - max_key_log2), where the x's
+// are the next four LCG states of the rank's sequence (which starts at the rank seed).
+__host__ __device__ __forceinline__ uint32_t npb_key(int r, int P, int64_t total, int64_t i,
+                                                     int max_key_log2) {
+    uint64_t x = npb_rank_seed(r, P, 4 * total);
+    x = (x * lcg_pow(kNpbA, 4ull * (uint64_t)i)) & kMask46;
+    uint64_t s = 0;
+    for (int j = 0; j < 4; j++) {
+        x = (x * kNpbA) & kMask46;
+        s += x;
+    }
+    return (uint32_t)(s >> (48 - max_key_log2));
+}
+
 __global__ void gen_keys_kernel(uint32_t* keys, int P, int64_t n, uint64_t seed, uint32_t step,
                                 int dist, int key_bits, const double* cdf, int64_t ncdf) {
     const int64_t total = (int64_t)P * n;
@@ -49,33 +143,8 @@ __global__ void gen_keys_kernel(uint32_t* keys, int P, int64_t n, uint64_t seed,
          g += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(g / n);
         const int64_t i = g - (int64_t)r * n;
-        uint32_t key;
-        if (dist == 0) {
-            u32x4v v = rank_stream(seed, r, step, (uint64_t)(i >> 2), kStreamKeys);
-            key = pick(v, (int)(i & 3)) >> (32 - key_bits);
-        } else if (dist == 1) {
-            u32x4v v = rank_stream(seed, r, step, (uint64_t)(i >> 1), kStreamZipf);
-            const int h = 2 * (int)(i & 1);
-            const double u = unit53(pick(v, h), pick(v, h + 1));
-            int64_t lo = 0, hi = ncdf - 1;
-            while (lo < hi) {
-                int64_t mid = lo + (hi - lo) / 2;
-                if (cdf[mid] > u) hi = mid; else lo = mid + 1;
-            }
-            key = (uint32_t)lo;
-        } else {
-            // NPB IS create_seq: key = (x1+x2+x3+x4) >> (48 - max_key_log2), where the x's
-            // are the next four LCG states of this rank's sequence (starts at the rank seed).
-            uint64_t x = npb_rank_seed(r, P, 4 * total);
-            x = (x * lcg_pow(kNpbA, 4ull * (uint64_t)i)) & kMask46;
-            uint64_t s = 0;
-            for (int j = 0; j < 4; j++) {
-                x = (x * kNpbA) & kMask46;
-                s += x;
-            }
-            key = (uint32_t)(s >> (48 - key_bits));
-        }
-        keys[g] = key;
+        keys[g] = dist == 2 ? npb_key(r, P, total, i, key_bits)
+                            : gen_key(dist, seed, r, step, i, key_bits, cdf, ncdf);
     }
 }
 

@@ -50,6 +50,17 @@ class ParticleArgs(ctypes.Structure):
                 ("seconds", _P)]
 
 
+class BucketSortArgs(ctypes.Structure):
+    """Bucket sort / NPB IS rank program: dist 0 uniform, 1 Zipf (cdf), 2 NPB IS class `cls`."""
+    _fields_ = [("dist", ctypes.c_int32), ("cls", ctypes.c_int32), ("key_bits", ctypes.c_int32),
+                ("bucket_bits", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("n", ctypes.c_int64), ("seed", ctypes.c_uint64),
+                ("cdf", _P), ("ncdf", ctypes.c_int64), ("kb2_cap", ctypes.c_int64),
+                ("count_cap", ctypes.c_int64), ("key_array", _P), ("key_buff1", _P),
+                ("key_buff2", _P), ("counting", _P), ("bsize", _P), ("btot", _P), ("cnt", _P),
+                ("nlocal", _P), ("passed", _P), ("total", _P), ("seconds", _P)]
+
+
 COLL_KINDS = {"bcast": 0, "reduce": 1, "allreduce": 2, "alltoall": 3, "allgather": 4,
               "gather": 5, "scatter": 6}
 
