@@ -337,12 +337,13 @@ def job_heap_bytes(job):
 
 
 class A2avPlan:
-    """Batched persistent-kernel Alltoallv step (gpumpi_a2av_*)."""
+    """Batched Alltoallv step: plan, scan and copy kernels, replayed as one hipGraph
+    (gpumpi_a2av_*)."""
 
     def __init__(self, P, elem_bytes, sendbufs_ptr, scounts_ptr, sdispls_ptr, recvbufs_ptr,
                  rcounts_ptr, rdispls_ptr, tune=None):
         self.lib = load_library()
-        t = A2avTune(0, 0, 0, -1) if tune is None else A2avTune(*tune)
+        t = A2avTune(0, -1, 0, -1) if tune is None else A2avTune(*tune)
         h = _P()
         rc = self.lib.gpumpi_a2av_plan_create(P, elem_bytes, sendbufs_ptr, scounts_ptr,
                                               sdispls_ptr, recvbufs_ptr, rcounts_ptr,
@@ -379,8 +380,8 @@ class A2avPlan:
         return d
 
     def phase_times(self, enable=None):
-        """enable=True arms/resets the kernel's phase timestamps, False disarms; with no
-        argument returns [phase1_us, barrier_us, total_us] of the last armed launch."""
+        """enable=True makes execute() time its kernels, False stops it; with no argument
+        returns [plan_us, scan_us, copy_us] of the last executed step."""
         if enable is not None:
             self.lib.gpumpi_a2av_phase_times(self.h, 1 if enable else 0, None)
             return None

@@ -632,55 +632,64 @@ static int cli_main(const void* kernel, int argc, const char* const* argv) {
 struct gpumpi_a2av_plan {
     A2avDev dev{};
     gpumpi_a2av_tune tune{};
-    int grid = 0;
+    int grid = 0;             // copy-kernel workgroups
     int cus = 0;
-    uint64_t last_waves = 0;
+    int64_t est_bytes = -1;   // payload of the last step read back (sizes the copy grid)
     std::vector<void*> allocs;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     int graph_steps = 0;
-    std::string err;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-static const void* a2av_kernel_for(int U, bool nt) {
+static const void* a2av_copy_for(int U, bool nt) {
     switch (U) {
-    case 2: return nt ? (const void*)&a2av_step_kernel<2, true> : (const void*)&a2av_step_kernel<2, false>;
-    case 4: return nt ? (const void*)&a2av_step_kernel<4, true> : (const void*)&a2av_step_kernel<4, false>;
-    case 8: return nt ? (const void*)&a2av_step_kernel<8, true> : (const void*)&a2av_step_kernel<8, false>;
-    default: return nt ? (const void*)&a2av_step_kernel<16, true> : (const void*)&a2av_step_kernel<16, false>;
+    case 1: return nt ? (const void*)&a2av_copy_kernel<1, true> : (const void*)&a2av_copy_kernel<1, false>;
+    case 2: return nt ? (const void*)&a2av_copy_kernel<2, true> : (const void*)&a2av_copy_kernel<2, false>;
+    case 4: return nt ? (const void*)&a2av_copy_kernel<4, true> : (const void*)&a2av_copy_kernel<4, false>;
+    default: return nt ? (const void*)&a2av_copy_kernel<8, true> : (const void*)&a2av_copy_kernel<8, false>;
     }
+}
+
+// Copy grid: one workgroup per chunk of the last measured step (P extra for the partial
+// last chunk of each row), or a default before any step was measured; blocks_per_cu > 0
+// caps it (the workgroups then stride over the chunks).
+static void a2av_size_grid(gpumpi_a2av_plan* pl) {
+    const int64_t chunk_bytes = 16ll * pl->dev.chunk_lines;
+    int64_t g = pl->est_bytes >= 0 ? pl->est_bytes / chunk_bytes + pl->dev.P : 64ll * pl->cus;
+    if (pl->tune.blocks_per_cu > 0 && g > (int64_t)pl->tune.blocks_per_cu * pl->cus)
+        g = (int64_t)pl->tune.blocks_per_cu * pl->cus;
+    if (g < 1) g = 1;
+    if (g > (1 << 22)) g = 1 << 22;
+    pl->grid = (int)g;
 }
 
 static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
-    int U = t.lines_per_lane;
-    if (U != 2 && U != 4 && U != 8 && U != 16) return GPUMPI_E_ARG;
-    int bt = t.block_threads;
-    if (bt != 256 && bt != 512 && bt != 1024) return GPUMPI_E_ARG;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a2av_kernel_for(U, t.nontemporal > 0), bt, 0) != hipSuccess ||
-        per_cu <= 0)
-        return GPUMPI_E_RESIDENCY;
-    int bpc = t.blocks_per_cu <= 0 ? 1 : t.blocks_per_cu;
-    if (bpc > per_cu) bpc = per_cu;  // the grid barrier needs every workgroup resident
+    const int U = t.lines_per_lane;
+    if (U != 1 && U != 2 && U != 4 && U != 8) return GPUMPI_E_ARG;
+    if (t.block_threads != kA2avCopyThreads || t.blocks_per_cu < 0) return GPUMPI_E_ARG;
     pl->tune = t;
-    pl->tune.blocks_per_cu = bpc;
-    pl->grid = bpc * pl->cus;
-    pl->dev.nontemporal = t.nontemporal > 0 ? 1 : 0;
+    pl->tune.nontemporal = t.nontemporal > 0 ? 1 : 0;
+    pl->dev.chunk_lines = (uint32_t)(kA2avCopyThreads * U);
+    a2av_size_grid(pl);
     return GPUMPI_OK;
 }
 
-static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s) {
-    const uint64_t waves = (uint64_t)pl->grid;
-    if (waves != pl->last_waves) {
-        // the arrival counter must be a multiple of the workgroup count at kernel start
-        hipLaunchKernelGGL(a2av_reset_kernel, dim3(1), dim3(64), 0, s, pl->dev.arrive,
-                           pl->dev.release);
-        pl->last_waves = waves;
-    }
+static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s, bool timed) {
+    const int P = pl->dev.P;
     void* kargs[] = {(void*)&pl->dev};
-    if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0), dim3(pl->grid),
-                        dim3(pl->tune.block_threads), kargs, 0, s) != hipSuccess)
+    if (timed) hipEventRecord(pl->ev[0], s);
+    if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(P), dim3(256), kargs, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
+    if (timed) hipEventRecord(pl->ev[1], s);
+    if (hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    if (timed) hipEventRecord(pl->ev[2], s);
+    if (hipLaunchKernel(a2av_copy_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0), dim3(pl->grid),
+                        dim3(kA2avCopyThreads), kargs, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    if (timed) hipEventRecord(pl->ev[3], s);
     return GPUMPI_OK;
 }
 
@@ -690,6 +699,12 @@ static void a2av_drop_graph(gpumpi_a2av_plan* pl) {
     pl->graph = nullptr;
     pl->graph_def = nullptr;
     pl->graph_steps = 0;
+}
+
+static int a2av_read_bytes(gpumpi_a2av_plan* pl, int64_t* bytes) {
+    if (hipMemcpy(bytes, pl->dev.total_bytes, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return GPUMPI_E_HIP;
+    return GPUMPI_OK;
 }
 
 extern "C" {
@@ -717,10 +732,8 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
     d.recvbufs = (char* const*)recvbufs;
     d.rcounts = recv_counts;
     d.rdispls = recv_displs;
-    d.timeout_ticks = (uint64_t)(20.0 * (double)khz * 1000.0);
     const uint64_t PP = (uint64_t)P * (uint64_t)P;
-    size_t ws = sizeof(A2avSeg) * PP + sizeof(uint64_t) * P + sizeof(uint64_t) * (P + 1) +
-                sizeof(int64_t) * P + 64 * 4;
+    const size_t ws = sizeof(uint64_t) * (2 * PP + P + (P + 1)) + 256;
     char* w = nullptr;
     if (hipMalloc((void**)&w, ws) != hipSuccess) {
         delete pl;
@@ -728,23 +741,16 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
     }
     pl->allocs.push_back(w);
     hipMemset(w, 0, ws);
-    d.table = (A2avSeg*)w;
-    w += sizeof(A2avSeg) * PP;
-    d.row_lines = (uint64_t*)w;
-    w += sizeof(uint64_t) * P;
-    d.row_off = (uint64_t*)w;
-    w += sizeof(uint64_t) * (P + 1);
-    d.row_bytes = (int64_t*)w;
-    w += sizeof(int64_t) * P;
-    d.total_bytes = (int64_t*)w;
-    d.arrive = (uint64_t*)(w + 64);
-    d.release = (uint64_t*)(w + 128);
-    d.error = (int32_t*)(w + 192);
-    d.stamps = nullptr;
-    gpumpi_a2av_tune t{512, 2, 8, 1};
+    d.seg_end = (uint64_t*)w;
+    d.seg_src = d.seg_end + PP;
+    d.row_bytes = d.seg_src + PP;
+    d.chunk_off = d.row_bytes + P;
+    d.total_bytes = (int64_t*)(d.chunk_off + P + 1);
+    for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
+    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1};
     if (tune) {
         if (tune->block_threads > 0) t.block_threads = tune->block_threads;
-        if (tune->blocks_per_cu > 0) t.blocks_per_cu = tune->blocks_per_cu;
+        if (tune->blocks_per_cu >= 0) t.blocks_per_cu = tune->blocks_per_cu;
         if (tune->lines_per_lane > 0) t.lines_per_lane = tune->lines_per_lane;
         if (tune->nontemporal >= 0) t.nontemporal = tune->nontemporal;
     }
@@ -762,13 +768,15 @@ void gpumpi_a2av_plan_destroy(gpumpi_a2av_plan* pl) {
     if (!pl) return;
     hipDeviceSynchronize();
     a2av_drop_graph(pl);
+    for (hipEvent_t e : pl->ev)
+        if (e) hipEventDestroy(e);
     for (void* p : pl->allocs) hipFree(p);
     delete pl;
 }
 
 int gpumpi_a2av_execute(gpumpi_a2av_plan* pl, void* stream) {
     if (!pl) return GPUMPI_E_ARG;
-    return a2av_launch(pl, (hipStream_t)stream);
+    return a2av_launch(pl, (hipStream_t)stream, pl->timing);
 }
 
 int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) {
@@ -776,31 +784,25 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) 
     hipStream_t s = (hipStream_t)stream;
     if (!pl->graph || pl->graph_steps != steps) {
         a2av_drop_graph(pl);
-        const uint64_t waves = (uint64_t)pl->grid;
-        if (waves != pl->last_waves) {
-            // reset outside the capture: no memset/reset node inside the graph
-            hipLaunchKernelGGL(a2av_reset_kernel, dim3(1), dim3(64), 0, s, pl->dev.arrive,
-                               pl->dev.release);
-            pl->last_waves = waves;
-        }
         hipStream_t cs;
         if (hipStreamCreate(&cs) != hipSuccess) return GPUMPI_E_HIP;
-        if (hipStreamSynchronize(s) != hipSuccess) return GPUMPI_E_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess) {
+            hipStreamDestroy(cs);
+            return GPUMPI_E_HIP;
+        }
         if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
             hipStreamDestroy(cs);
             return GPUMPI_E_HIP;
         }
         int rc = GPUMPI_OK;
-        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
-            void* kargs[] = {(void*)&pl->dev};
-            if (hipLaunchKernel(a2av_kernel_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0), dim3(pl->grid),
-                                dim3(pl->tune.block_threads), kargs, 0, cs) != hipSuccess)
-                rc = GPUMPI_E_HIP;
-        }
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) rc = a2av_launch(pl, cs, false);
         hipGraph_t g = nullptr;
         hipError_t ee = hipStreamEndCapture(cs, &g);
         hipStreamDestroy(cs);
-        if (rc != GPUMPI_OK || ee != hipSuccess) return GPUMPI_E_HIP;
+        if (rc != GPUMPI_OK || ee != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return GPUMPI_E_HIP;
+        }
         pl->graph_def = g;
         if (hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0) != hipSuccess) {
             a2av_drop_graph(pl);
@@ -813,34 +815,17 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) 
 
 int gpumpi_a2av_phase_times(gpumpi_a2av_plan* pl, int32_t enable, double* us) {
     if (!pl) return GPUMPI_E_ARG;
-    if (enable > 0) {
-        if (!pl->dev.stamps) {
-            void* q = nullptr;
-            if (hipMalloc(&q, 64) != hipSuccess) return GPUMPI_E_NOMEM;
-            pl->allocs.push_back(q);
-            pl->dev.stamps = (uint64_t*)q;
-            a2av_drop_graph(pl);
-        }
-        uint64_t init[4] = {~0ull, 0, 0, 0};
-        if (hipMemcpy(pl->dev.stamps, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess)
-            return GPUMPI_E_HIP;
+    if (enable >= 0) {
+        pl->timing = enable > 0;
         return GPUMPI_OK;
     }
-    if (enable == 0) {
-        pl->dev.stamps = nullptr;
-        a2av_drop_graph(pl);
-        return GPUMPI_OK;
+    if (!pl->timing || !us) return GPUMPI_E_ARG;
+    if (hipEventSynchronize(pl->ev[3]) != hipSuccess) return GPUMPI_E_HIP;
+    for (int i = 0; i < 3; i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pl->ev[i], pl->ev[i + 1]) != hipSuccess) return GPUMPI_E_HIP;
+        us[i] = 1000.0 * ms;
     }
-    if (!pl->dev.stamps || !us) return GPUMPI_E_ARG;
-    uint64_t st[4];
-    if (hipMemcpy(st, pl->dev.stamps, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
-        return GPUMPI_E_HIP;
-    int32_t khz = 0;
-    gpumpi_device_info(nullptr, &khz, nullptr);
-    const double tick_us = 1000.0 / (double)khz;
-    us[0] = (double)(st[1] - st[0]) * tick_us;  // phase 1 (slowest wave)
-    us[1] = (double)(st[2] - st[0]) * tick_us;  // through the grid barrier
-    us[2] = (double)(st[3] - st[0]) * tick_us;  // whole step
     return GPUMPI_OK;
 }
 
@@ -853,55 +838,52 @@ int gpumpi_a2av_get_tune(gpumpi_a2av_plan* pl, gpumpi_a2av_tune* t, int32_t* gri
 
 int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* pl, int64_t* bytes) {
     if (!pl || !bytes) return GPUMPI_E_ARG;
-    int32_t err = 0;
-    if (hipMemcpy(bytes, pl->dev.total_bytes, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&err, pl->dev.error, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
-        return GPUMPI_E_HIP;
-    return err ? GPUMPI_E_DEVICE : GPUMPI_OK;
+    return a2av_read_bytes(pl, bytes);
 }
 
-// Sweep launch shapes on the live buffers; every candidate runs the real step (so the
-// receive buffers hold the correct exchange afterwards).
+// Sweep chunk sizes, copy-grid caps and store kinds on the live buffers; every candidate
+// runs the real step (so the receive buffers hold the correct exchange afterwards).  The
+// first step's payload sizes the copy grid.
 int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* chosen,
                          double* best_us) {
     if (!pl) return GPUMPI_E_ARG;
     hipStream_t s = (hipStream_t)stream;
     a2av_drop_graph(pl);
-    const int threads[] = {256, 512, 1024};
-    const int bpcs[] = {1, 2, 4, 8};
-    const int us[] = {4, 8, 16};
+    if (a2av_launch(pl, s, false) != GPUMPI_OK || hipStreamSynchronize(s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    int64_t bytes = 0;
+    if (a2av_read_bytes(pl, &bytes) != GPUMPI_OK) return GPUMPI_E_HIP;
+    pl->est_bytes = bytes;
+    const int bpcs[] = {0, 64, 16, 4};
+    const int us_[] = {1, 2, 4, 8};
     const int nts[] = {1, 0};
     gpumpi_a2av_tune best = pl->tune;
     double best_t = 1e30;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (int bt : threads)
-        for (int bpc : bpcs) {
-            if ((int64_t)bt * bpc > 2048) continue;  // at most 32 waves per CU
-            for (int U : us)
-                for (int nt : nts) {
-                    gpumpi_a2av_tune t{bt, bpc, U, nt};
-                    if (a2av_configure(pl, t) != GPUMPI_OK) continue;
-                    if (pl->tune.blocks_per_cu != bpc) continue;  // occupancy-clamped duplicate
-                    a2av_launch(pl, s);  // warm-up
-                    const int reps = 3;
-                    hipEventRecord(e0, s);
-                    for (int r = 0; r < reps; r++) a2av_launch(pl, s);
-                    hipEventRecord(e1, s);
-                    hipEventSynchronize(e1);
-                    float ms = 0.f;
-                    hipEventElapsedTime(&ms, e0, e1);
-                    double t_us = 1000.0 * ms / reps;
-                    if (t_us < best_t) {
-                        best_t = t_us;
-                        best = pl->tune;
-                    }
+    for (int bpc : bpcs)
+        for (int U : us_)
+            for (int nt : nts) {
+                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt};
+                if (a2av_configure(pl, t) != GPUMPI_OK) continue;
+                a2av_launch(pl, s, false);  // warm-up
+                const int reps = 5;
+                hipEventRecord(e0, s);
+                for (int r = 0; r < reps; r++) a2av_launch(pl, s, false);
+                hipEventRecord(e1, s);
+                hipEventSynchronize(e1);
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                const double t_us = 1000.0 * ms / reps;
+                if (t_us < best_t) {
+                    best_t = t_us;
+                    best = pl->tune;
                 }
-        }
+            }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    int rc = a2av_configure(pl, best);
+    const int rc = a2av_configure(pl, best);
     if (chosen) *chosen = pl->tune;
     if (best_us) *best_us = best_t;
     return rc;

@@ -1,4 +1,4 @@
-// Batched MPI_Alltoallv step: one persistent kernel per step.
+// Batched MPI_Alltoallv step: three kernels replayed as one hipGraph per step.
 //
 // Semantics: gpu_mpi/mpi.cu:363-408 (block s of rank d's receive buffer <- block d of
 // rank s's send buffer, counts/displacements in elements), preceded by the Alltoall of
@@ -6,45 +6,47 @@
 // between the two calls (toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI/IS/is.c:758-781).
 //
 // The reference runs this as P-1 Isend + P-1 Irecv per rank, each payload copied by one
-// thread.  Here the whole step is decoupled from ranks:
+// thread.  Here the step is decoupled from ranks and split at the only global dependency
+// (the receive displacements need every sender's count):
 //
-//  phase 1  one wavefront per destination row d: recv_counts[d][s] = send_counts[s][d],
-//           recv_displs = exclusive scan over s, and a segment table entry per (d, s):
-//           {src, dst, bytes, first 16-byte line of the row}; lines are counted in
-//           absolute 16-byte destination lines, so every full line is one aligned store.
-//  barrier  one arrival per workgroup; the last workgroup to arrive scans the per-row
-//           line totals into row offsets and releases the others (one grid barrier, no
-//           second kernel).
-//  phase 2  the global line space is split evenly over ALL wavefronts of the grid, so a
-//           Zipf-skewed receiver is copied by many CUs at once.  A wavefront keeps a
-//           window of 64 segment records in its lanes (compacted to the non-empty
-//           ones), and each lane locates its line's
-//           segment: a wave-uniform cursor plus one compare selects between the (at
-//           most two) segments of a 64-line slice, with a 6-step shuffle search only
-//           for slices crossing several tiny segments.  U lines per lane are loaded
-//           before any is stored (16 KiB in flight per wave at U=16).  Full lines are
-//           unaligned 16-byte global loads and 16-byte aligned, optionally
-//           non-temporal stores; the <=2 partial lines per segment are copied bytewise
-//           by the lane that holds the segment.
+//  plan   one workgroup per destination row d: recv_counts[d][s] = send_counts[s][d],
+//         recv_displs = exclusive scan over s, and per segment (d, s) its end offset in
+//         the row and its source address.
+//  scan   one workgroup: chunks per row (a row's destination lines, counted from the
+//         128-byte block holding its first byte, cut into chunks of chunk_lines 16-byte
+//         lines) and their exclusive prefix.
+//  copy   one workgroup per chunk (grid-stride when the grid is smaller): wave 0 finds
+//         the chunk's row and first segment (64-ary searches over L2-resident tables)
+//         and stages a window of 64 segments in LDS; each wave then copies a contiguous
+//         run of 64*U lines slice by slice, keeping a wave-uniform cursor into the window
+//         (it only moves forward) and choosing each lane's segment among the cursor's next
+//         four with compares.  A line inside one segment is one unaligned 16-byte load
+//         and one aligned 16-byte store; a line that straddles segments is four dword
+//         loads (issued with the other lines' loads) and one 16-byte store.  Lines at a row's ends, and dwords that are split
+//         or unaligned at the source, go byte by byte; bytes outside the row are never
+//         written.
 //
-// HBM traffic per step = payload read once + payload written once + the O(P^2) tables.
+// Why not one persistent kernel with a grid barrier: on gfx950 an agent-scope release or
+// acquire writes back or invalidates the XCD's L2 (buffer_wbl2 / buffer_inv sc1); a grid
+// barrier across 256-2048 workgroups measured 14-150 us, a kernel boundary inside a graph
+// a few us.  And a copy grid of many short workgroups streams faster than a persistent
+// grid (5.9 vs 5.0-5.2 TB/s for a plain 512 MiB copy on MI355X).
+//
+// HBM traffic per step = payload read once + payload written once + O(P^2) tables
+// (counts and displacements read, 4 + 4 + 8 + 8 bytes per segment written and read).
 #pragma once
 
 #include "copy.h"
 
 namespace gpumpi {
 
-struct A2avSeg {
-    const char* src;
-    char* dst;
-    uint64_t bytes;
-    uint64_t first;  // first line of this segment, relative to its row
-};
+constexpr int kA2avWindow = 64;
+constexpr int kA2avCopyThreads = 256;
 
 struct A2avDev {
     int32_t P;
     int32_t elem;
-    int32_t nontemporal;
+    uint32_t chunk_lines;   // destination lines per chunk = kA2avCopyThreads * U
     int32_t pad;
     char* const* sendbufs;
     const int32_t* scounts;
@@ -52,24 +54,12 @@ struct A2avDev {
     char* const* recvbufs;
     int32_t* rcounts;
     int32_t* rdispls;
-    A2avSeg* table;         // [P*P]
-    uint64_t* row_lines;    // [P]
-    uint64_t* row_off;      // [P+1] exclusive prefix, written by the last phase-1 wave
-    int64_t* row_bytes;     // [P]
+    uint64_t* seg_end;      // [P*P] end byte offset of segment (d, s) within row d
+    uint64_t* seg_src;      // [P*P] address of the first source byte of segment (d, s)
+    uint64_t* row_bytes;    // [P]
+    uint64_t* chunk_off;    // [P+1] exclusive prefix of chunks per row; [P] = total chunks
     int64_t* total_bytes;   // [1]
-    uint64_t* arrive;       // grid barrier arrivals, one per workgroup (monotonic across launches)
-    uint64_t* release;      // grid barrier release epoch
-    int32_t* error;         // set on a timed-out barrier
-    uint64_t timeout_ticks;
-    uint64_t* stamps;       // optional: [start min, phase-1 end max, barrier exit max, end max]
 };
-
-__device__ __forceinline__ void stamp_min(uint64_t* a, uint64_t v) {
-    __hip_atomic_fetch_min(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void stamp_max(uint64_t* a, uint64_t v) {
-    __hip_atomic_fetch_max(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __device__ __forceinline__ int64_t wave_excl_scan(int64_t v, int64_t* total) {
     int64_t inc = wave_incl_scan(v);
@@ -77,20 +67,18 @@ __device__ __forceinline__ int64_t wave_excl_scan(int64_t v, int64_t* total) {
     return inc - v;
 }
 
-// Largest i in [0, n) with a[i] <= x (a non-decreasing, a[0] <= x); wave-uniform.
-__device__ __forceinline__ int64_t wave_search_le(const uint64_t* a, int64_t n, uint64_t x,
-                                                  int64_t stride_elems = 1, int64_t base = 0) {
+// Largest i in [0, n) with a[i] <= x (a non-decreasing, a[0] <= x); wave-uniform,
+// 64-ary: each step compares 64 evenly spaced entries.
+__device__ __forceinline__ int64_t wave_search_le(const uint64_t* a, int64_t n, uint64_t x) {
     int64_t lo = 0, hi = n;
     while (hi - lo > 1) {
-        int64_t span = hi - lo;
-        int64_t step = (span + 63) / 64;
-        int64_t idx = lo + (int64_t)lane() * step;
-        bool le = false;
-        if (idx < hi) le = a[base + idx * stride_elems] <= x;
-        uint64_t m = ballot(le);
-        int L = m ? 63 - __builtin_clzll(m) : 0;
-        int64_t nlo = lo + (int64_t)L * step;
-        int64_t nhi = nlo + step;
+        const int64_t step = (hi - lo + 63) / 64;
+        const int64_t idx = lo + (int64_t)lane() * step;
+        const bool le = idx < hi && a[idx] <= x;
+        const uint64_t m = ballot(le);
+        const int L = m ? 63 - __builtin_clzll(m) : 0;
+        const int64_t nlo = lo + (int64_t)L * step;
+        const int64_t nhi = nlo + step;
         lo = nlo;
         hi = nhi < hi ? nhi : hi;
     }
@@ -102,265 +90,356 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int k) {
     uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), k);
     return ((uint64_t)hi << 32) | lo;
 }
-// Forward permute of a 64-bit value: lane i's value lands in lane dst_i.
-__device__ __forceinline__ uint64_t push64(uint64_t v, int dst) {
-    uint32_t lo = (uint32_t)__builtin_amdgcn_ds_permute(dst << 2, (int)(uint32_t)v);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_ds_permute(dst << 2, (int)(uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
-// Copies destination lines [line_begin, line_end) of the global line space.
-// Each segment k of the current window is described by scalars held in lane k:
-//   G  first global line            A  = (dst >> 4) - G   (line j starts at (A + j) * 16)
-//   D  = src - dst                  LO = dst, HI = dst + bytes
-template <int U, bool NT>
-__device__ void a2av_copy_lines(const A2avDev& p, uint64_t line_begin, uint64_t line_end) {
-    const int P = p.P;
-    const int64_t nseg = (int64_t)P * P;
-    const int l = lane();
-    if (line_begin >= line_end) return;
-    int64_t d = wave_search_le(p.row_off, P, line_begin);
-    const uint64_t roff = p.row_off[d];
-    int64_t s = wave_search_le((const uint64_t*)&p.table[0].first, P, line_begin - roff,
-                               (int64_t)(sizeof(A2avSeg) / sizeof(uint64_t)),
-                               (int64_t)d * P * (int64_t)(sizeof(A2avSeg) / sizeof(uint64_t)));
-    int64_t seg_base = d * P + s;
-    uint64_t line = line_begin;
-    const uint64_t lt_mask = l == 0 ? 0ull : (~0ull >> (64 - l));
-    while (line < line_end && seg_base < nseg) {
-        // ---- load a window of 64 segments, keep the non-empty ones in lanes 0..cnt-1
-        const int64_t f = seg_base + l;
-        uint64_t G = ~0ull, A = 0, D = 0, LO = 0, HI = 0, nl = 0;
-        if (f < nseg) {
-            const A2avSeg sg = p.table[f];
-            const uint64_t dst = (uint64_t)(uintptr_t)sg.dst;
-            G = p.row_off[f / P] + sg.first;
-            nl = sg.bytes ? ((dst + sg.bytes + 15) >> 4) - (dst >> 4) : 0;
-            A = (dst >> 4) - G;
-            D = (uint64_t)(uintptr_t)sg.src - dst;
-            LO = dst;
-            HI = dst + sg.bytes;
-        }
-        const bool ne = nl != 0;
-        const uint64_t m = ballot(ne);
-        const int cnt = __popcll(m);
-        seg_base += 64;
-        if (cnt == 0) continue;
-        const int to = ne ? __popcll(m & lt_mask) : cnt + __popcll(~m & lt_mask);
-        G = push64(ne ? G : ~0ull, to);
-        A = push64(A, to);
-        D = push64(D, to);
-        LO = push64(LO, to);
-        HI = push64(HI, to);
-        const int last = cnt - 1;
-        // end line of the window = first line of the last segment + its line count
-        const uint64_t g_last = readlane64(G, last);
-        const uint64_t hi_last = readlane64(HI, last), lo_last = readlane64(LO, last);
-        const uint64_t win_end = g_last + (((hi_last + 15) >> 4) - (lo_last >> 4));
-        const uint64_t lim = win_end < line_end ? win_end : line_end;
-        // partial first/last lines of each segment: copied bytewise by the segment's lane
-        // when the line belongs to [line, lim), the part of the window this wave owns
-        if (l < cnt) {
-            const uint64_t nlk = ((HI + 15) >> 4) - (LO >> 4);
-            for (int e = 0; e < 2; e++) {
-                const uint64_t jl = e == 0 ? G : G + nlk - 1;
-                if (e == 1 && nlk == 1) break;
-                if (jl < line || jl >= lim) continue;
-                const uint64_t ls = (A + jl) << 4;
-                if (ls >= LO && ls + 16 <= HI) continue;  // full line: main loop
-                const uint64_t b0 = ls > LO ? ls : LO;
-                const uint64_t b1 = ls + 16 < HI ? ls + 16 : HI;
-                char* dp = (char*)(uintptr_t)b0;
-                const char* sp = (const char*)(uintptr_t)(b0 + D);
-                const uint64_t nb = b1 - b0;
-                if (((b0 | (uint64_t)(uintptr_t)sp | nb) & 3) == 0) {
-                    for (uint64_t q = 0; q < nb; q += 4)
-                        *(uint32_t*)(dp + q) = *(const uint32_t*)(sp + q);
-                } else {
-                    for (uint64_t q = 0; q < nb; q++) dp[q] = sp[q];
-                }
-            }
-        }
-        int kcur = 0;
-        while (line < lim) {
-            u32x4 v[U];
-            uint64_t da[U];
-            bool full[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint64_t L0 = line + (uint64_t)u * kWave;
-                full[u] = false;
-                da[u] = 0;
-                if (L0 >= lim) continue;  // wave-uniform
-                // advance the current segment to the one holding line L0 (uniform loop)
-                while (kcur < last && readlane64(G, kcur + 1) <= L0) kcur++;
-                const uint64_t j = L0 + (uint64_t)l;
-                const uint64_t g1 = kcur < last ? readlane64(G, kcur + 1) : ~0ull;
-                const uint64_t g2 = kcur + 1 < last ? readlane64(G, kcur + 2) : ~0ull;
-                uint64_t a, dd, lo, hi;
-                if (L0 + (kWave - 1) < g2) {
-                    // fast path: the 64-line slice spans at most two segments
-                    const bool sel = j >= g1;
-                    const int k1 = kcur < last ? kcur + 1 : kcur;
-                    a = sel ? readlane64(A, k1) : readlane64(A, kcur);
-                    dd = sel ? readlane64(D, k1) : readlane64(D, kcur);
-                    lo = sel ? readlane64(LO, k1) : readlane64(LO, kcur);
-                    hi = sel ? readlane64(HI, k1) : readlane64(HI, kcur);
-                } else {
-                    // slow path (many tiny segments): per-lane search over the window
-                    const uint64_t jj = j < lim ? j : L0;
-                    int k = 0;
-#pragma unroll
-                    for (int b = 32; b > 0; b >>= 1) {
-                        uint64_t g = __shfl(G, k + b);
-                        if (k + b <= last && g <= jj) k += b;
-                    }
-                    a = __shfl(A, k);
-                    dd = __shfl(D, k);
-                    lo = __shfl(LO, k);
-                    hi = __shfl(HI, k);
-                }
-                const uint64_t ls = (a + j) << 4;  // line start address
-                full[u] = j < lim && ls >= lo && ls + 16 <= hi;
-                da[u] = ls;
-                if (full[u]) v[u] = gload16(ls + dd);
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++)
-                if (full[u]) gstore16<NT>(da[u], v[u]);
-            line += (uint64_t)U * kWave;
-        }
-        line = lim;
-    }
-}
-
-template <int U, bool NT>
-__global__ void __launch_bounds__(1024) a2av_step_kernel(A2avDev p) {
+// ------------------------------------------------------------------------------ plan
+// One workgroup of four wavefronts per destination row d; wave w owns a contiguous
+// quarter of the sources.  Pass 1 sums the quarter's counts (the waves' totals give each
+// quarter's starting displacement), pass 2 scans and writes.
+__global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
+    __shared__ int64_t s_tot[4];
     const int P = p.P;
     const int l = lane();
-    const uint32_t wpb = blockDim.x >> 6;
-    const uint64_t W = (uint64_t)gridDim.x * wpb;
-    const uint64_t w = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
-    if (p.stamps && threadIdx.x == 0) stamp_min(&p.stamps[0], wall_clock64());
-
-    // ---------------- phase 1: counts transpose, displacements, segment table
-    for (int64_t d = (int64_t)w; d < P; d += (int64_t)W) {
-        int64_t carry = 0;
-        uint64_t carry_lines = 0;
-        char* rbase = p.recvbufs[d];
-        for (int s0 = 0; s0 < P; s0 += 4 * kWave) {
-            // issue the strided column loads of up to 4 chunks before any scan
-            int64_t c[4], sd[4];
-            const char* sb[4];
+    const int w = threadIdx.x >> 6;
+    const int64_t d = blockIdx.x;
+    const int Q = (((P + 3) / 4) + kWave - 1) / kWave * kWave;
+    const int sb = w * Q;
+    const int se = sb + Q < P ? sb + Q : P;
+    int64_t part = 0;
+    for (int s0 = sb; s0 < se; s0 += 4 * kWave) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int s = s0 + q * kWave + l;
-                const bool valid = s < P;
-                c[q] = valid ? p.scounts[(int64_t)s * P + d] : 0;
-                sd[q] = valid ? p.sdispls[(int64_t)s * P + d] : 0;
-                sb[q] = valid ? p.sendbufs[s] : nullptr;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int s = s0 + q * kWave + l;
-                if (s0 + q * kWave >= P) break;
-                const bool valid = s < P;
-                int64_t tot;
-                int64_t rd = wave_excl_scan(c[q], &tot) + carry;
-                carry += tot;
-                uint64_t nb = (uint64_t)c[q] * (uint64_t)p.elem;
-                char* dst = rbase + rd * p.elem;
-                uint64_t lines = nb ? (((uint64_t)(uintptr_t)dst + nb + 15) >> 4) -
-                                          ((uint64_t)(uintptr_t)dst >> 4)
-                                    : 0;
-                int64_t ltot;
-                uint64_t lfirst = (uint64_t)wave_excl_scan((int64_t)lines, &ltot) + carry_lines;
-                carry_lines += (uint64_t)ltot;
-                if (valid) {
-                    p.rcounts[d * P + s] = (int32_t)c[q];
-                    p.rdispls[d * P + s] = (int32_t)rd;
-                    A2avSeg sg;
-                    sg.src = sb[q] + sd[q] * p.elem;
-                    sg.dst = dst;
-                    sg.bytes = nb;
-                    sg.first = lfirst;
-                    p.table[d * P + s] = sg;
-                }
-            }
-        }
-        if (l == 0) {
-            p.row_lines[d] = carry_lines;
-            p.row_bytes[d] = carry * p.elem;
+        for (int q = 0; q < 4; q++) {
+            const int s = s0 + q * kWave + l;
+            part += s < se ? p.scounts[(int64_t)s * P + d] : 0;
         }
     }
-
-    // ---------------- grid barrier, one arrival per workgroup; the last arriving
-    // workgroup's first wavefront builds the row offsets and releases the others
-    __shared__ uint32_t s_last;
-    __shared__ uint64_t s_epoch;
-    fence_release();
-    if (p.stamps && l == 0) stamp_max(&p.stamps[1], wall_clock64());
+    part = wave_sum(part);
+    if (l == 0) s_tot[w] = part;
     __syncthreads();
+    int64_t carry = 0, row = 0;
+    for (int k = 0; k < 4; k++) {
+        carry += k < w ? s_tot[k] : 0;
+        row += s_tot[k];
+    }
+    for (int s0 = sb; s0 < se; s0 += 4 * kWave) {
+        // the strided column loads of up to four 64-source chunks go out before any scan
+        int64_t c[4], sd[4];
+        const char* sbuf[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int s = s0 + q * kWave + l;
+            const bool valid = s < se;
+            c[q] = valid ? p.scounts[(int64_t)s * P + d] : 0;
+            sd[q] = valid ? p.sdispls[(int64_t)s * P + d] : 0;
+            sbuf[q] = valid ? p.sendbufs[s] : nullptr;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (s0 + q * kWave >= se) break;
+            const int s = s0 + q * kWave + l;
+            int64_t tot;
+            const int64_t rd = wave_excl_scan(c[q], &tot) + carry;
+            carry += tot;
+            if (s < se) {
+                const int64_t o = d * P + s;
+                p.rcounts[o] = (int32_t)c[q];
+                p.rdispls[o] = (int32_t)rd;
+                p.seg_end[o] = (uint64_t)(rd + c[q]) * (uint64_t)p.elem;
+                p.seg_src[o] = (uint64_t)(uintptr_t)(sbuf[q] + sd[q] * p.elem);
+            }
+        }
+    }
+    if (threadIdx.x == 0) p.row_bytes[d] = (uint64_t)row * (uint64_t)p.elem;
+}
+
+// ------------------------------------------------------------------------------ scan
+__global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
+    __shared__ uint64_t part[16];
+    __shared__ int64_t bpart[16];
+    const int P = p.P;
+    const int w = threadIdx.x >> 6;
+    const uint64_t C = p.chunk_lines;
+    uint64_t carry = 0;
+    int64_t bytes = 0;
+    for (int base = 0; base < P; base += 1024) {
+        const int d = base + (int)threadIdx.x;
+        uint64_t ch = 0;
+        int64_t nb = 0;
+        if (d < P) {
+            nb = (int64_t)p.row_bytes[d];
+            const uint64_t rb = (uint64_t)(uintptr_t)p.recvbufs[d];
+            const uint64_t lines = nb ? ((rb + (uint64_t)nb + 15) >> 4) - ((rb >> 7) << 3) : 0;
+            ch = (lines + C - 1) / C;
+        }
+        const uint64_t inc = (uint64_t)wave_incl_scan((int64_t)ch);
+        const int64_t bsum = wave_sum(nb);
+        if (lane() == 63) part[w] = inc;
+        if (lane() == 0) bpart[w] = bsum;
+        __syncthreads();
+        uint64_t before = 0, total = 0;
+        for (int k = 0; k < 16; k++) {
+            if (k < w) before += part[k];
+            total += part[k];
+            bytes += (threadIdx.x == 0) ? bpart[k] : 0;
+        }
+        if (d < P) p.chunk_off[d] = carry + before + inc - ch;
+        carry += total;
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        const uint64_t B = gridDim.x;
-        const uint64_t old = fetch_add(p.arrive, (uint64_t)1);
-        s_epoch = old / B + 1;
-        s_last = (old % B == B - 1) ? 1u : 0u;
+        p.chunk_off[P] = carry;
+        *p.total_bytes = bytes;
     }
-    __syncthreads();
-    const uint64_t epoch = s_epoch;
-    if (s_last) {
-        if (threadIdx.x < 64) {
-            fence_acquire();
-            uint64_t carry = 0;
-            int64_t bytes = 0;
-            for (int d0 = 0; d0 < P; d0 += kWave) {
-                const int d = d0 + l;
-                int64_t v = d < P ? (int64_t)p.row_lines[d] : 0;
-                int64_t b = d < P ? p.row_bytes[d] : 0;
-                int64_t tot;
-                int64_t ex = wave_excl_scan(v, &tot);
-                if (d < P) p.row_off[d] = carry + (uint64_t)ex;
-                carry += (uint64_t)tot;
-                bytes += wave_sum(b);
+}
+
+// ------------------------------------------------------------------------------ copy
+// One segment seen from its row: bytes [st, en) of the row; the source of the row byte at
+// offset q is (rbase + q) + D.
+struct A2avSegView {
+    int64_t st, en;
+    uint64_t D;
+};
+
+// Segment holding row offset q (0 <= q < row bytes), by binary search over the row's
+// segment ends in HBM (the path for lines outside the LDS window).
+__device__ __forceinline__ A2avSegView a2av_seg_global(const A2avDev& p, int64_t d, uint64_t rbase,
+                                                       int64_t q) {
+    const uint64_t* se = p.seg_end + d * p.P;
+    int64_t lo = 0, hi = p.P - 1;  // first s with se[s] > q
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)se[mid] > q) hi = mid; else lo = mid + 1;
+    }
+    A2avSegView v;
+    v.st = lo > 0 ? (int64_t)se[lo - 1] : 0;
+    v.en = (int64_t)se[lo];
+    v.D = p.seg_src[d * p.P + lo] - (rbase + (uint64_t)v.st);
+    return v;
+}
+
+// Window entry holding row offset q (s_st[0] <= q < coverage): largest k < cnt with
+// s_st[k] <= q.  cnt is wave-uniform, so the search takes ceil(log2(cnt)) steps.
+__device__ __forceinline__ int a2av_seg_lds(const int64_t* s_st, int cnt, int64_t q) {
+    int k = 0;
+    for (int b = cnt > 1 ? 1 << (31 - __builtin_clz(cnt - 1)) : 0; b > 0; b >>= 1)
+        if (k + b < cnt && s_st[k + b] <= q) k += b;
+    return k;
+}
+
+struct A2avWin {
+    const int64_t* st;
+    const int64_t* en;
+    const uint64_t* D;
+    int cnt;
+    int nsearch;        // entries starting before the pass's last line: the search range
+    int64_t lo, cover;  // row offsets [lo, cover) are described by the window
+};
+
+__device__ __forceinline__ A2avSegView a2av_seg_at(const A2avDev& p, int64_t d, uint64_t rbase,
+                                                   const A2avWin& w, int64_t q) {
+    if (q >= w.lo && q < w.cover) {
+        const int k = a2av_seg_lds(w.st, w.nsearch, q);
+        return A2avSegView{w.st[k], w.en[k], w.D[k]};
+    }
+    return a2av_seg_global(p, d, rbase, q);
+}
+
+// A destination line that straddles segments or the row's ends: assembled per dword
+// (per byte where a dword is split or its source unaligned); only row bytes are written.
+__device__ void a2av_boundary_line(const A2avDev& p, int64_t d, uint64_t rbase, int64_t rbytes,
+                                   const A2avWin& w, uint64_t line) {
+    const uint64_t la = line << 4;
+    const int64_t off = (int64_t)la - (int64_t)rbase;
+    uint32_t v[4] = {0, 0, 0, 0};
+    int mask = 0;
+    for (int i = 0; i < 4; i++) {
+        const int64_t q = off + 4 * i;
+        if (q >= 0 && q + 4 <= rbytes) {
+            const A2avSegView sg = a2av_seg_at(p, d, rbase, w, q);
+            const uint64_t src = la + 4 * i + sg.D;
+            if (q + 4 <= sg.en && (src & 3) == 0) {
+                v[i] = *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)src;
+                mask |= 1 << i;
+                continue;
             }
+        }
+        for (int j = 0; j < 4; j++) {
+            const int64_t b = q + j;
+            if (b < 0 || b >= rbytes) continue;
+            const A2avSegView sg = a2av_seg_at(p, d, rbase, w, b);
+            const uint64_t a = la + 4 * i + j;
+            *(GPUMPI_GLOBAL uint8_t*)(uintptr_t)a = *(const GPUMPI_GLOBAL uint8_t*)(uintptr_t)(a + sg.D);
+        }
+    }
+    if (mask == 0xF) {
+        *(GPUMPI_GLOBAL u32x4*)(uintptr_t)la = u32x4{v[0], v[1], v[2], v[3]};
+    } else {
+        for (int i = 0; i < 4; i++)
+            if (mask & (1 << i)) *(GPUMPI_GLOBAL uint32_t*)(uintptr_t)(la + 4 * i) = v[i];
+    }
+}
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) {
+    __shared__ int64_t s_st[kA2avWindow];
+    __shared__ int64_t s_en[kA2avWindow];
+    __shared__ uint64_t s_D[kA2avWindow];
+    __shared__ int64_t s_meta[10];
+    const int P = p.P;
+    const int tid = threadIdx.x;
+    const int l = lane();
+    const uint64_t C = p.chunk_lines;
+    const uint64_t total = p.chunk_off[P];
+    for (uint64_t ch = blockIdx.x; ch < total; ch += gridDim.x) {
+        // ---- the chunk's row and line range
+        if (tid < kWave) {
+            const int64_t d = wave_search_le(p.chunk_off, P, ch);
+            const uint64_t c = ch - p.chunk_off[d];
+            const uint64_t rbase = (uint64_t)(uintptr_t)p.recvbufs[d];
+            const uint64_t rbytes = p.row_bytes[d];
+            const uint64_t lend = (rbase + rbytes + 15) >> 4;
+            // a row's lines start at the 128-byte block holding its first byte, so every
+            // wave's 1 KiB slice covers whole 128-byte blocks (lines before the row's first
+            // byte write nothing)
+            const uint64_t line0 = ((rbase >> 7) << 3) + c * C;
+            const uint64_t line1 = line0 + C < lend ? line0 + C : lend;
             if (l == 0) {
-                p.row_off[P] = carry;
-                *p.total_bytes = bytes;
+                s_meta[0] = d;
+                s_meta[1] = (int64_t)rbase;
+                s_meta[2] = (int64_t)rbytes;
+                s_meta[3] = (int64_t)line0;
+                s_meta[4] = (int64_t)line1;
             }
-            fence_release();
-            if (l == 0) st_release(p.release, epoch);
         }
-    } else if (threadIdx.x == 0) {
-        const uint64_t t0 = wall_clock64();
-        uint32_t spins = 0;
-        for (;;) {
-            if (ld_acquire(p.release) >= epoch) break;
-            if ((++spins & 255) == 0 && wall_clock64() - t0 > p.timeout_ticks) {
-                st_relaxed(p.error, 1);
-                s_last = 2u;  // abandon the step
-                break;
+        __syncthreads();
+        const int64_t d = s_meta[0];
+        const uint64_t rbase = (uint64_t)s_meta[1];
+        const int64_t rbytes = s_meta[2];
+        const uint64_t line1 = (uint64_t)s_meta[4];
+        uint64_t pos = (uint64_t)s_meta[3];
+        while (pos < line1) {
+            // ---- window: 64 segments from the one holding the row offset of line `pos`
+            if (tid < kWave) {
+                int64_t o = (int64_t)(pos << 4) - (int64_t)rbase;
+                if (o < 0) o = 0;
+                const uint64_t* se = p.seg_end + d * P;
+                int64_t s0 = 0;
+                if ((int64_t)se[0] <= o) s0 = wave_search_le(se, P, (uint64_t)o) + 1;
+                const int64_t s = s0 + l;
+                const bool valid = s < P;
+                const int64_t en = valid ? (int64_t)se[s] : 0;
+                const int64_t st = valid ? (s > 0 ? (int64_t)se[s - 1] : 0) : 0;
+                const uint64_t src = valid ? p.seg_src[d * P + s] : 0;
+                const bool ne = valid && en > st;
+                const uint64_t m = ballot(ne);
+                const int k = __popcll(m & (l == 0 ? 0ull : (~0ull >> (64 - l))));
+                if (ne) {
+                    s_st[k] = st;
+                    s_en[k] = en;
+                    s_D[k] = src - (rbase + (uint64_t)st);
+                }
+                const int64_t cover =
+                    s0 + kWave >= P ? rbytes : (int64_t)readlane64((uint64_t)en, kWave - 1);
+                // the pass takes the lines that end at or before the coverage, at least one
+                uint64_t pend = (rbase + (uint64_t)cover) >> 4;
+                if (pend > line1) pend = line1;
+                int64_t slow = 0;
+                if (pend <= pos) {
+                    pend = pos + 1;
+                    slow = 1;
+                }
+                // entries that start before the end of the pass's lines bound the searches
+                const int64_t pend_off = (int64_t)(pend << 4) - (int64_t)rbase;
+                const uint64_t mb = ballot(ne && st < pend_off);
+                if (l == 0) {
+                    s_meta[5] = __popcll(m);
+                    s_meta[6] = (int64_t)pend;
+                    s_meta[7] = slow ? 0 : cover;
+                    s_meta[8] = slow ? 0 : o;
+                    s_meta[9] = __popcll(mb) > 0 ? __popcll(mb) : 1;
+                }
             }
-            __builtin_amdgcn_s_sleep(1);
+            __syncthreads();
+            A2avWin w;
+            w.st = s_st;
+            w.en = s_en;
+            w.D = s_D;
+            w.cnt = (int)s_meta[5];
+            w.nsearch = (int)s_meta[9];
+            const uint64_t pend = (uint64_t)s_meta[6];
+            w.cover = s_meta[7];
+            w.lo = s_meta[8];
+            // each wave copies a contiguous run of 64*U lines, slice by slice, with a
+            // wave-uniform cursor over the window (segments only move forward)
+            const int wv = tid >> 6;
+            for (uint64_t base = pos; base < pend; base += (uint64_t)kA2avCopyThreads * U) {
+                const uint64_t wb = base + (uint64_t)wv * kWave * U;
+                if (wb >= pend) continue;
+                int64_t off0 = (int64_t)(wb << 4) - (int64_t)rbase;
+                int kc = a2av_seg_lds(s_st, w.nsearch, off0 > w.lo ? off0 : w.lo);
+                u32x4 v[U];
+                int kind[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint64_t sl = wb + (uint64_t)u * kWave;  // slice's first line
+                    const uint64_t line = sl + l;
+                    kind[u] = line < pend ? 2 : 0;
+                    if (sl >= pend) continue;  // wave-uniform
+                    const int64_t soff = (int64_t)(sl << 4) - (int64_t)rbase;
+                    // move the cursor to the entry holding the slice's first byte: a few
+                    // steps forward, or a search when the slice starts further ahead
+                    if (kc + 3 < w.nsearch && soff >= s_en[kc + 3])
+                        kc = a2av_seg_lds(s_st, w.nsearch, soff);
+                    else
+                        while (kc + 1 < w.nsearch && soff >= s_en[kc]) kc++;
+                    const int64_t off = soff + 16 * l;
+                    if (!(line < pend && off >= w.lo && off + 16 <= w.cover)) continue;
+                    // this lane's entry: the cursor or one of the next three, else a search
+                    int kb = kc;
+                    if (kc + 3 < w.cnt && off >= s_en[kc + 3]) kb = a2av_seg_lds(s_st, w.nsearch, off);
+                    const int k1 = kb + 1 < w.cnt ? kb + 1 : kb, k2 = kb + 2 < w.cnt ? kb + 2 : k1,
+                              k3 = kb + 3 < w.cnt ? kb + 3 : k2;
+                    const int64_t e0 = s_en[kb], e1 = s_en[k1], e2 = s_en[k2], e3 = s_en[k3];
+                    const uint64_t D0 = s_D[kb], D1 = s_D[k1], D2 = s_D[k2], D3 = s_D[k3];
+                    const uint64_t la = line << 4;
+                    const int j = (off >= e0) + (off >= e1) + (off >= e2);
+                    const int64_t ej = j == 0 ? e0 : j == 1 ? e1 : j == 2 ? e2 : e3;
+                    const uint64_t Dj = j == 0 ? D0 : j == 1 ? D1 : j == 2 ? D2 : D3;
+                    if (off + 16 <= ej) {
+                        v[u] = gload16(la + Dj);
+                        kind[u] = 1;
+                        continue;
+                    }
+                    // the line straddles segments: one dword from the segment holding it
+                    uint64_t a[4];
+                    bool ok = true;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int64_t q = off + 4 * i;
+                        const int ji = (q >= e0) + (q >= e1) + (q >= e2);
+                        const int64_t eq = ji == 0 ? e0 : ji == 1 ? e1 : ji == 2 ? e2 : e3;
+                        const uint64_t Dq = ji == 0 ? D0 : ji == 1 ? D1 : ji == 2 ? D2 : D3;
+                        a[i] = la + 4 * i + Dq;
+                        ok = ok && (a[i] & 3) == 0 && q + 4 <= eq;
+                    }
+                    if (ok) {
+                        v[u] = u32x4{*(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)a[0],
+                                     *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)a[1],
+                                     *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)a[2],
+                                     *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)a[3]};
+                        kind[u] = 1;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (kind[u] == 1) gstore16<NT>((wb + (uint64_t)u * kWave + l) << 4, v[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (kind[u] == 2)
+                        a2av_boundary_line(p, d, rbase, rbytes, w, wb + (uint64_t)u * kWave + l);
+            }
+            __syncthreads();
+            pos = pend;
         }
-    }
-    __syncthreads();
-    if (s_last == 2u) return;
-    fence_acquire();
-    if (p.stamps && threadIdx.x == 0) stamp_max(&p.stamps[2], wall_clock64());
-
-    // ---------------- phase 2: even split of all destination lines over all waves
-    const uint64_t L = p.row_off[P];
-    const uint64_t b = L * w / W;          // L < 2^40 lines and W < 2^20 waves
-    const uint64_t e = L * (w + 1) / W;
-    a2av_copy_lines<U, NT>(p, b, e);
-    if (p.stamps && l == 0) stamp_max(&p.stamps[3], wall_clock64());
-}
-
-__global__ void a2av_reset_kernel(uint64_t* arrive, uint64_t* release) {
-    if (threadIdx.x == 0) {
-        *arrive = 0;
-        *release = 0;
     }
 }
 

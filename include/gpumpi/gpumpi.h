@@ -106,7 +106,7 @@ int gpumpi_heap_stats(gpumpi_runtime* rt, int64_t* in_use, int64_t* peak, int64_
 int gpumpi_device_info(int32_t* num_cus, int32_t* wallclock_khz, int32_t* max_waves_per_cu);
 
 /* ------------------------------------------------------------------------------------
- * Batched Alltoallv step (persistent kernel, hipGraph replay, autotuned launch shape).
+ * Batched Alltoallv step (plan + scan + copy kernels, hipGraph replay, autotuned shape).
  *
  * Describes one MPI_Alltoallv over P ranks whose buffers are already resident in HBM:
  *   sendbufs[P], recvbufs[P]        device arrays of device pointers
@@ -114,16 +114,19 @@ int gpumpi_device_info(int32_t* num_cus, int32_t* wallclock_khz, int32_t* max_wa
  *   send_displs[P*P]                device int32, [s][d] = element displacement in s's sendbuf
  * The step performs the Alltoall of the counts (recv_counts[d][s] = send_counts[s][d]),
  * the receive displacements (exclusive prefix over s, written to recv_displs[d][s]) and
- * the payload exchange, all in ONE kernel launch.  recv_counts/recv_displs are device
+ * the payload exchange: three dependent kernels (plan, scan, copy) that graph_replay
+ * captures once and replays as one hipGraph launch.  recv_counts/recv_displs are device
  * int32 [P*P] outputs.  elem_bytes is the datatype size.
  * ------------------------------------------------------------------------------------ */
 typedef struct gpumpi_a2av_plan gpumpi_a2av_plan;
 
 typedef struct gpumpi_a2av_tune {
-    int32_t block_threads;      /* 256 / 512 / 1024; 0 = autotune */
-    int32_t blocks_per_cu;      /* persistent blocks per CU; 0 = autotune */
-    int32_t lines_per_lane;     /* 16-byte vectors in flight per lane: 2,4,8; 0 = autotune */
-    int32_t nontemporal;        /* 1 = non-temporal stores, 0 = cached, -1 = autotune */
+    int32_t block_threads;      /* copy workgroup size: 256 (0 = default) */
+    int32_t blocks_per_cu;      /* copy workgroups per CU at most; 0 = one per chunk,
+                                   -1 = default */
+    int32_t lines_per_lane;     /* 16-byte lines per thread and chunk: 1, 2, 4, 8
+                                   (chunk = 256 * lines); 0 = default */
+    int32_t nontemporal;        /* 1 = non-temporal stores, 0 = cached, -1 = default */
 } gpumpi_a2av_tune;
 
 int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,
@@ -139,11 +142,11 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* plan, int32_t steps_per_graph, vo
 /* Sweep launch shapes on the live buffers, keep the fastest; returns the chosen shape. */
 int gpumpi_a2av_autotune(gpumpi_a2av_plan* plan, void* stream, gpumpi_a2av_tune* chosen,
                          double* best_us);
-/* Phase timestamps of the step kernel (profiling): enable = 1 arms and resets them,
- * 0 disarms, -1 reads us[3] = {phase 1 done, grid barrier passed, step done} in
- * microseconds since the first workgroup started (maxima over the last armed launch). */
+/* Per-kernel durations of a step (profiling): enable = 1 makes gpumpi_a2av_execute record
+ * HIP events around its kernels, 0 stops it, -1 reads us[3] = {plan, scan, copy} in
+ * microseconds for the last executed step (graph replays are not timed). */
 int gpumpi_a2av_phase_times(gpumpi_a2av_plan* plan, int32_t enable, double* us);
-/* Launch shape in use and the number of workgroups of the persistent grid. */
+/* Launch shape in use and the number of workgroups of the copy grid. */
 int gpumpi_a2av_get_tune(gpumpi_a2av_plan* plan, gpumpi_a2av_tune* tune, int32_t* grid);
 /* Total payload bytes moved by the last completed step (reads them back from the device). */
 int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* plan, int64_t* bytes);
