@@ -1,15 +1,17 @@
 #!/usr/bin/env python3
-"""Benchmark: batched persistent-kernel MPI_Alltoallv on one MI355X.
+"""Benchmark: batched MPI_Alltoallv on one MI355X.
 
 Workload (the bucket-sort exchange): P = 512 ranks, 2^18 uint32 keys per rank (2^27
 keys, 512 MiB) uniform in [0, 2^23), bucketed NPB-style into 1024 buckets (NPB IS
-rank(): histogram, splitters, stable bucketing).  One timed step = the batched exchange
-kernel: Alltoall of the counts + receive displacements + the skewed Alltoallv of the
-keys, one persistent kernel replayed from a hipGraph.  Inputs are resident in HBM
+rank(): histogram, splitters, stable bucketing).  One timed step = the batched exchange:
+Alltoall of the counts + receive displacements + the skewed Alltoallv of the keys, as
+three kernels (plan, scan, copy) replayed from one hipGraph.  Inputs are resident in HBM
 before the timed region.
 
 value  = algorithmic bytes / time = 2 x payload bytes (read once + written once) per
          step / seconds per step, in GB/s.
+roofline = the copy kernel alone (HIP events around it on its stream, averaged): the
+         same algorithmic bytes / its duration, against the 8 TB/s HBM peak.
 Also reported: the plain per-call path (one rank-grid kernel launch per MPI call:
 MPI_Alltoall then MPI_Alltoallv on 512 wavefront ranks) and the ratio, the Zipf(1.1)
 variant, the host-MPI CPU baseline (the same exchange under MPICH, one rank per host
@@ -131,6 +133,20 @@ def time_steps(torch, fn, steps):
     return e0.elapsed_time(e1) / steps  # ms per step
 
 
+def kernel_times(torch, plan, stream, reps):
+    """Average per-kernel durations of a step (HIP events recorded by the library around
+    the plan, scan and copy kernels on the step's stream)."""
+    plan.phase_times(True)
+    acc = [0.0, 0.0, 0.0]
+    for _ in range(reps):
+        plan.execute(stream)
+        t = plan.phase_times()
+        acc = [a + b for a, b in zip(acc, t)]
+    plan.phase_times(False)
+    torch.cuda.synchronize()
+    return {"plan_us": acc[0] / reps, "scan_us": acc[1] / reps, "copy_us": acc[2] / reps}
+
+
 def cpu_baseline(args):
     """The same exchange under host MPI (MPICH), one rank per host core, timed by the
     oracle's MPI program.  Returns the cpu_baseline object or None."""
@@ -186,12 +202,13 @@ def make_jobs(pkg, njobs, seed, heap_budget):
 
 def serve_latency(pkg, torch, args):
     """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
-    the host, for the device-resident server and for one plain rank-grid launch per job."""
+    the host, for one plain rank-grid launch per job and for the device-resident server.
+    The plain phases run first (no server grid resident), then one server serves every
+    arrival rate; each served result is compared with the same job run alone."""
     import numpy as np
     heap = 16 << 30
     budget = int(0.9 * heap)
     jobs = make_jobs(pkg, args.serve_jobs, 5, budget)
-    # plain path capacity: the jobs back to back
     rt = pkg.Runtime(256, heap_bytes=heap, timeout_s=60.0)
     res = torch.zeros(len(jobs) * 64, dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
@@ -204,36 +221,26 @@ def serve_latency(pkg, torch, args):
     torch.cuda.synchronize()
     plain_capacity = len(jobs) / (time.perf_counter() - t0)
     plain_raw = res.cpu().numpy().reshape(len(jobs), 64)
+    plain_res = [pkg.JobResult.from_buffer_copy(plain_raw[k].tobytes()) for k in range(len(jobs))]
+    plain_failed = sum(1 for p in plain_res if p.status != 1)
+    log(f"serve: {len(jobs)} jobs, plain capacity {plain_capacity:.1f} jobs/s")
     out = {"jobs": len(jobs), "plain_capacity_jobs_per_s": round(plain_capacity, 1), "rates": []}
-    srv = pkg.Server(args.serve_slots, queue_capacity=8192, max_job_ranks=256,
-                     heap_fraction=0.9, ranks_per_block=4, heap_bytes=heap, timeout_s=60.0)
-    mismatches = 0
-    for frac in args.serve_rates:
-        lam = frac * plain_capacity
+
+    def arrivals_for(frac):
         rng = np.random.default_rng(int(frac * 1000))
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(jobs)))
-        # ---- served
-        lat_s = {}
-        start = time.perf_counter()
-        i = 0
-        while len(lat_s) < len(jobs):
-            now = time.perf_counter() - start
-            while i < len(jobs) and arrivals[i] <= now:
-                if not srv.submit(jobs[i]):
-                    break
-                i += 1
-            for r in srv.poll():
-                lat_s[r.id] = (time.perf_counter() - start) - arrivals[r.id]
-                p = pkg.JobResult.from_buffer_copy(plain_raw[r.id].tobytes())
-                if r.status != 1 or (r.dot != p.dot) or (r.checksum != p.checksum):
-                    mismatches += 1
-            if now - arrivals[-1] > 120:
-                break
-        served_span = time.perf_counter() - start
-        # ---- plain: each job launched as its own rank grid at its arrival time
+        return np.cumsum(rng.exponential(1.0 / (frac * plain_capacity), size=len(jobs)))
+
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 3) if len(v) else None
+
+    # ---- plain: each job launched as its own rank grid at its arrival time
+    plain_rows = {}
+    for frac in args.serve_rates:
+        arrivals = arrivals_for(frac)
         events = [torch.cuda.Event() for _ in jobs]
         lat_p = {}
         start = time.perf_counter()
+        last_log = start
         i = 0
         pending = []
         while len(lat_p) < len(jobs):
@@ -250,26 +257,60 @@ def serve_latency(pkg, torch, args):
                 else:
                     still.append(k)
             pending = still
+            if time.perf_counter() - last_log > 10:
+                last_log = time.perf_counter()
+                log(f"serve plain {frac}x: {len(lat_p)}/{len(jobs)} done")
             if now - arrivals[-1] > 120:
                 break
-        plain_span = time.perf_counter() - start
-        ls = np.array(list(lat_s.values())) * 1e3
+        span = time.perf_counter() - start
         lp = np.array(list(lat_p.values())) * 1e3
-        row = {"rate_fraction": frac, "rate_jobs_per_s": round(lam, 1),
-               "served": {"p50_ms": round(float(np.percentile(ls, 50)), 3),
-                          "p99_ms": round(float(np.percentile(ls, 99)), 3),
-                          "jobs_per_s": round(len(ls) / served_span, 1)},
-               "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 3),
-                         "p99_ms": round(float(np.percentile(lp, 99)), 3),
-                         "jobs_per_s": round(len(lp) / plain_span, 1)}}
-        log(f"serve rate {frac}x ({lam:.0f}/s): served p50 {row['served']['p50_ms']} ms "
-            f"p99 {row['served']['p99_ms']} ms; plain p50 {row['plain']['p50_ms']} ms "
-            f"p99 {row['plain']['p99_ms']} ms")
-        out["rates"].append(row)
-    stats = srv.stop()
+        plain_rows[frac] = {"p50_ms": pct(lp, 50), "p99_ms": pct(lp, 99),
+                            "jobs_per_s": round(len(lp) / span, 1), "completed": len(lp)}
+    torch.cuda.synchronize()
     rt.close()
-    out["server_stats"] = stats
+
+    # ---- served: one device-resident server for every rate
+    srv = pkg.Server(args.serve_slots, queue_capacity=8192, max_job_ranks=256,
+                     heap_fraction=0.9, ranks_per_block=4, heap_bytes=heap, timeout_s=60.0)
+    mismatches, served_failed = 0, 0
+    for frac in args.serve_rates:
+        arrivals = arrivals_for(frac)
+        lat_s = {}
+        start = time.perf_counter()
+        last_log = start
+        i = 0
+        while len(lat_s) < len(jobs):
+            now = time.perf_counter() - start
+            while i < len(jobs) and arrivals[i] <= now:
+                if not srv.submit(jobs[i]):
+                    break
+                i += 1
+            for r in srv.poll():
+                lat_s[r.id] = (time.perf_counter() - start) - arrivals[r.id]
+                p = plain_res[r.id]
+                if r.status != 1:
+                    served_failed += 1
+                elif p.status == 1 and (r.dot != p.dot or r.checksum != p.checksum):
+                    mismatches += 1
+            if time.perf_counter() - last_log > 10:
+                last_log = time.perf_counter()
+                log(f"serve served {frac}x: {len(lat_s)}/{len(jobs)} done")
+            if now - arrivals[-1] > 120:
+                break
+        span = time.perf_counter() - start
+        ls = np.array(list(lat_s.values())) * 1e3
+        row = {"rate_fraction": frac, "rate_jobs_per_s": round(frac * plain_capacity, 1),
+               "served": {"p50_ms": pct(ls, 50), "p99_ms": pct(ls, 99),
+                          "jobs_per_s": round(len(ls) / span, 1), "completed": len(ls)},
+               "plain": plain_rows[frac]}
+        log(f"serve rate {frac}x ({frac * plain_capacity:.0f}/s): served p50 "
+            f"{row['served']['p50_ms']} ms p99 {row['served']['p99_ms']} ms; plain p50 "
+            f"{row['plain']['p50_ms']} ms p99 {row['plain']['p99_ms']} ms")
+        out["rates"].append(row)
+    out["server_stats"] = srv.stop()
     out["result_mismatches_vs_plain"] = mismatches
+    out["served_failed"] = served_failed
+    out["plain_failed"] = plain_failed
     return out
 
 
@@ -328,6 +369,11 @@ def main():
         f"tune {plan.tune()}")
     ex.verify(torch)
     tune_used = plan.tune()
+    # the dominant kernel (copy) alone: HIP events around it on its own stream, averaged
+    kern = kernel_times(torch, plan, stream, max(5, args.steps // 5))
+    copy_gbs = algo / (kern["copy_us"] * 1e-6) / 1e9
+    log(f"kernels: plan {kern['plan_us']:.1f} us, scan {kern['scan_us']:.1f} us, copy "
+        f"{kern['copy_us']:.1f} us = {copy_gbs:.1f} GB/s ({100*copy_gbs/HBM_PEAK_GBS:.1f}% of peak)")
 
     # ---------------- plain per-call path (one rank-grid launch per MPI call)
     plain = None
@@ -402,13 +448,15 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (Philox uniform keys, NPB-style buckets)",
-        "config": {"workload": "bucket-sort exchange: 512 ranks x 2^18 uint32 keys (2^27 total), "
+        "config": {"workload": f"bucket-sort exchange: {P} ranks x 2^{args.keys_log2} uint32 keys, "
                                "uniform [0,2^23), 1024 buckets; one step = Alltoall(counts) + "
-                               "Alltoallv(keys) in one persistent kernel, hipGraph replay",
+                               "Alltoallv(keys) as plan/scan/copy kernels replayed as one hipGraph",
                    "ranks": P, "keys_per_rank": n, "payload_bytes": ex.payload,
-                   "tune": tune_used},
-        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+                   "tune": tune_used,
+                   "kernel_us": {k: round(v, 2) for k, v in kern.items()}},
+        "roofline": {"bound": "hbm", "kernel": "a2av_copy_kernel",
+                     "achieved": round(copy_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(copy_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "plain_path": plain,
         "batched_vs_plain": round(plain["ms_per_step"] / ms, 2) if plain else None,

@@ -52,6 +52,7 @@ __global__ void init_runtime_kernel(Runtime* R, InitBases b) {
         s.scratch = b.scratch + r * kScratchBytes;
         s.initialized = 0;
         s.finalized = 0;
+        s.arena = nullptr;
         for (int c = 0; c < kHeapSmallClasses; c++) s.cache.n[c] = 0;
         R->mailbox[r].tail = 0;
         R->world->slots[r].seq = 0;
@@ -106,6 +107,19 @@ __global__ void resize_world_kernel(Runtime* R, int k) {
         R->world_group->size = k;
         R->world->arrive.value = 0;
         R->world->gen.value = 0;
+    }
+}
+
+// Empties the heap: page bump, class stacks, the ranks' block caches, live-byte count.
+// Launched before each plain job, which -- like a fresh program -- starts on a clean heap.
+__global__ void heap_reset_kernel(Runtime* R, int nstates) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int r = tid; r < nstates; r += gridDim.x * blockDim.x)
+        for (int c = 0; c < kHeapSmallClasses; c++) R->ranks[r].cache.n[c] = 0;
+    if (tid < kHeapClasses) R->heap.free_head[tid] = kHeapEmpty;
+    if (tid == 0) {
+        R->heap.page_bump = 0;
+        R->heap.bytes_in_use = 0;
     }
 }
 
@@ -998,10 +1012,12 @@ int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t
               halloc((void**)&h.q_tail, 64) &&
               halloc((void**)&h.done, sizeof(progs::JobResult) * queue_capacity) &&
               halloc((void**)&h.done_count, 64) && halloc((void**)&h.stop, 64) &&
+              halloc((void**)&h.heartbeat, 64) &&
               dalloc((void**)&h.dq, sizeof(progs::JobDesc) * queue_capacity) &&
               dalloc((void**)&h.boxes, sizeof(progs::SlotBox) * S) &&
               dalloc((void**)&h.jobs, sizeof(progs::JobSlot) * jcap) &&
               dalloc((void**)&h.free_slots, sizeof(int32_t) * S) &&
+              dalloc((void**)&h.free_jobs, sizeof(int32_t) * jcap) &&
               dalloc((void**)&h.slot_lists, sizeof(int32_t) * (size_t)jcap * max_job_ranks) &&
               dalloc((void**)&h.exit_flag, 64) && dalloc((void**)&h.stats, 64) &&
               dalloc((void**)&sv->d_sh, sizeof(progs::ServeShared));
@@ -1013,7 +1029,43 @@ int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t
     h.jcap = jcap;
     h.max_ranks = max_job_ranks;
     h.lookahead = 16;
-    h.heap_budget = (int64_t)((double)sv->rt->h.heap.bytes * heap_fraction);
+    // the job heap: ntop blocks of 2^(16 + top) bytes, top chosen so that a quarter of the
+    // budget fits in one block
+    {
+        const uint64_t budget = (uint64_t)((double)cfg->heap_bytes * heap_fraction);
+        int top = 0;
+        while (top < progs::kBuddyOrders - 1 && ((uint64_t)1 << (progs::kBuddyMinShift + top)) < budget / 4) top++;
+        uint64_t ntop = budget >> (progs::kBuddyMinShift + top);
+        if (ntop < 1) ntop = 1;
+        progs::Buddy& b = h.jobheap;
+        b.top = top;
+        b.ntop = (int32_t)ntop;
+        uint64_t words = 0;
+        for (int o = 0; o <= top; o++) {
+            b.off[o] = words;
+            words += (((uint64_t)ntop << (top - o)) + 63) / 64;
+        }
+        const uint64_t region = ntop << (progs::kBuddyMinShift + top);
+        ok = dalloc((void**)&b.base, region) && dalloc((void**)&b.bits, 8 * words) &&
+             dalloc((void**)&b.order_of, (size_t)ntop << top);
+        if (ok) {
+            std::vector<uint64_t> topbits((ntop + 63) / 64, 0);
+            for (uint64_t i = 0; i < ntop; i++) topbits[i >> 6] |= 1ull << (i & 63);
+            ok = hipMemcpy(b.bits + b.off[top], topbits.data(), 8 * topbits.size(),
+                           hipMemcpyHostToDevice) == hipSuccess;
+        }
+        h.heap_budget = (int64_t)1 << (progs::kBuddyMinShift + top);
+    }
+    if (!ok) {
+        gpumpi_serve_stop(sv, nullptr);
+        return GPUMPI_E_NOMEM;
+    }
+    {
+        int32_t khz = 0;
+        gpumpi_device_info(nullptr, &khz, nullptr);
+        const double idle_s = cfg->timeout_s > 0 ? cfg->timeout_s : 20.0;
+        h.idle_ticks = (uint64_t)(idle_s * (double)khz * 1000.0);
+    }
     if (hipMemcpy(sv->d_sh, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreate(&sv->stream) != hipSuccess) {
         gpumpi_serve_stop(sv, nullptr);
@@ -1038,11 +1090,13 @@ int gpumpi_serve_submit(gpumpi_server* sv, const gpumpi_job* job) {
     sv->sh.q[sv->submitted % (uint64_t)sv->qcap] = d;
     sv->submitted++;
     __atomic_store_n(sv->sh.q_tail, sv->submitted, __ATOMIC_RELEASE);
+    __atomic_fetch_add(sv->sh.heartbeat, 1, __ATOMIC_RELEASE);
     return GPUMPI_OK;
 }
 
 int gpumpi_serve_poll(gpumpi_server* sv, gpumpi_job_result* out, int32_t max) {
     if (!sv || (!out && max > 0)) return GPUMPI_E_ARG;
+    if (sv->running) __atomic_fetch_add(sv->sh.heartbeat, 1, __ATOMIC_RELEASE);
     const uint64_t done = __atomic_load_n(sv->sh.done_count, __ATOMIC_ACQUIRE);
     int n = 0;
     while (sv->polled < done && n < max) {
@@ -1097,6 +1151,7 @@ int gpumpi_run_job(gpumpi_runtime* rt, const gpumpi_job* job, gpumpi_job_result*
         return GPUMPI_E_ARG;
     hipStream_t s = stream ? (hipStream_t)stream : rt->own_stream;
     hipLaunchKernelGGL(resize_world_kernel, dim3(4), dim3(256), 0, s, rt->d_rt, job->ranks);
+    hipLaunchKernelGGL(heap_reset_kernel, dim3(4), dim3(256), 0, s, rt->d_rt, rt->cfg.num_ranks);
     rt->world_size = job->ranks;
     progs::JobArgs a;
     a.job = progs::JobDesc{job->id, job->seed, job->type, job->ranks, job->size};

@@ -45,12 +45,13 @@ __host__ __device__ __forceinline__ int64_t pow2_at_least(int64_t v) {
     while (p < v) p <<= 1;
     return p;
 }
-// Heap bytes a job holds at its peak, rounded up to the heap's power-of-two classes
-// (two buffers per rank), plus one communicator object.
+// Heap bytes a job holds: its arena (two buffers per rank in 256-byte granules), which
+// the server allocates whole at admission.
 __host__ __device__ __forceinline__ int64_t job_bytes(const JobDesc& j) {
     const int64_t per = j.type == 0 ? ((j.size + j.ranks - 1) / j.ranks) * 8
                                     : (int64_t)j.ranks * j.size;
-    return 2 * pow2_at_least(per) * j.ranks + pow2_at_least(64 * 1024 + 96 * (int64_t)j.ranks);
+    const int64_t g = ((per > 0 ? per : 1) + 255) & ~(int64_t)255;
+    return 2 * g * j.ranks;
 }
 
 // Runs one rank of a job on communicator `comm`; rank 0 fills *res (value fields).
@@ -139,6 +140,77 @@ __device__ int w_job(int, char**) {
 }
 
 // ---------------------------------------------------------------- served mode
+// The server's job heap: a buddy allocator over one region of ntop blocks of 2^top bytes
+// (smallest block 64 KiB).  Only the scheduler wave uses it, so it needs no atomics, and
+// a freed block merges with its free buddy all the way up -- admission asks it for the
+// job's whole arena, so "fits" means "fits now, contiguously".  Every lane applies the
+// same updates (each lane reads back its own stores).
+constexpr int kBuddyMinShift = 16;
+constexpr int kBuddyOrders = 40;
+
+struct Buddy {
+    char* base;
+    int32_t top;                     // order of the largest blocks (block = 2^(16 + order))
+    int32_t ntop;                    // number of largest blocks
+    uint64_t* bits;                  // free bitmaps of all orders
+    uint64_t off[kBuddyOrders];      // word offset of order o's bitmap in bits
+    uint8_t* order_of;               // [ntop << top] order of the allocated block at each 64 KiB
+};
+
+__device__ __forceinline__ bool buddy_test(const Buddy& b, int o, uint64_t i) {
+    return (b.bits[b.off[o] + (i >> 6)] >> (i & 63)) & 1ull;
+}
+__device__ __forceinline__ void buddy_set(const Buddy& b, int o, uint64_t i, bool v) {
+    uint64_t* w = &b.bits[b.off[o] + (i >> 6)];
+    const uint64_t m = 1ull << (i & 63);
+    const uint64_t x = *w;
+    *w = v ? (x | m) : (x & ~m);
+}
+// First free block of order o (64 bitmap words per step across the lanes), or -1.
+__device__ int64_t buddy_find(const Buddy& b, int o) {
+    const uint64_t nblk = (uint64_t)b.ntop << (b.top - o);
+    const uint64_t nw = (nblk + 63) / 64;
+    for (uint64_t w0 = 0; w0 < nw; w0 += kWave) {
+        const uint64_t wi = w0 + lane();
+        const uint64_t x = wi < nw ? b.bits[b.off[o] + wi] : 0;
+        const uint64_t m = ballot(x != 0);
+        if (m) {
+            const int L = __builtin_ctzll(m);
+            const uint64_t xv = readlane64(x, L);
+            return (int64_t)((w0 + L) * 64 + __builtin_ctzll(xv));
+        }
+    }
+    return -1;
+}
+__device__ char* buddy_alloc(const Buddy& b, uint64_t need) {
+    int o = 0;
+    while (((uint64_t)1 << (kBuddyMinShift + o)) < need) o++;
+    if (o > b.top) return nullptr;
+    for (int k = o; k <= b.top; k++) {
+        int64_t i = buddy_find(b, k);
+        if (i < 0) continue;
+        buddy_set(b, k, (uint64_t)i, false);
+        for (int j = k - 1; j >= o; j--) {  // split: keep the left half, free the right
+            i = 2 * i;
+            buddy_set(b, j, (uint64_t)i + 1, true);
+        }
+        b.order_of[(uint64_t)i << o] = (uint8_t)o;
+        return b.base + ((uint64_t)i << (kBuddyMinShift + o));
+    }
+    return nullptr;
+}
+__device__ void buddy_free(const Buddy& b, char* p) {
+    const uint64_t b0 = (uint64_t)(p - b.base) >> kBuddyMinShift;
+    int o = b.order_of[b0];
+    uint64_t i = b0 >> o;
+    while (o < b.top && buddy_test(b, o, i ^ 1)) {  // merge with a free buddy
+        buddy_set(b, o, i ^ 1, false);
+        i >>= 1;
+        o++;
+    }
+    buddy_set(b, o, i, true);
+}
+
 struct SlotBox {
     uint32_t seq;       // incremented by the scheduler for every assignment
     int32_t job;        // index into the job table
@@ -153,6 +225,7 @@ struct JobSlot {
     int32_t remaining;  // ranks still running (atomic)
     int32_t state;      // 0 free, 1 running, 2 finished (set by the last rank)
     int32_t* slots;     // the job's world slots (scheduler-owned list)
+    JobArena arena;     // the job's memory, allocated at admission
 };
 
 struct ServeShared {
@@ -163,22 +236,28 @@ struct ServeShared {
     JobResult* done;             // [qcap] completion ring written by the device
     uint64_t* done_count;        // number of completions (device release)
     int32_t* stop;               // host sets 1: finish admitted jobs and exit
+    uint64_t* heartbeat;         // bumped by every host submit/poll call
     // device memory
     SlotBox* boxes;              // [S]
     JobSlot* jobs;               // [jcap] running-job table
     int32_t* free_slots;         // [S] stack of idle worker slots (scheduler-owned)
+    int32_t* free_jobs;          // [jcap] stack of free job-table entries (scheduler-owned)
     int32_t* slot_lists;         // [jcap * max_ranks] world slots of each running job
     int32_t* exit_flag;          // scheduler tells workers to leave
     int64_t* stats;              // [4]: admitted, rejected, max concurrent jobs, backfills
     int32_t qcap, jcap, max_ranks, lookahead;
-    int64_t heap_budget;         // admission limit on reserved heap bytes
+    int64_t heap_budget;         // largest arena a job can ever get (2^top)
+    Buddy jobheap;               // the job heap (scheduler-owned)
+    uint64_t idle_ticks;         // no heartbeat for this long: behave as if stop was set
 };
 
+// Polls of host-written words: coherent loads without the cache invalidate of an acquire;
+// the caller fences once it acts on a new value.
 __device__ __forceinline__ uint64_t sys_load(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ int32_t sys_load32(const int32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ void serve_worker(const ServeShared& S) {
@@ -188,8 +267,8 @@ __device__ void serve_worker(const ServeShared& S) {
         uint32_t seq = 0;
         int32_t ex = 0;
         if (lane() == 0) {
-            seq = ld_acquire(&box->seq);
-            ex = ld_acquire(S.exit_flag);
+            seq = ld_relaxed(&box->seq);
+            ex = ld_relaxed(S.exit_flag);
         }
         seq = bcast(seq);
         ex = bcast(ex);
@@ -198,7 +277,9 @@ __device__ void serve_worker(const ServeShared& S) {
             fence_acquire();
             const int jb = box->job;
             JobSlot* js = &S.jobs[jb];
+            my_state()->arena = &js->arena;
             int rc = job_rank(js->desc, js->comm, &js->res);
+            my_state()->arena = nullptr;
             if (rc != MPI_SUCCESS) js->res.status = -2;
             heap_flush_cache(&rt()->heap, &my_state()->cache);
             fence_release();
@@ -224,53 +305,77 @@ __device__ void serve_scheduler(const ServeShared& S) {
     int nfree = 0;
     for (int s = 1 + lane(); s < nslots; s += kWave) S.free_slots[s - 1] = s;
     nfree = nslots - 1;
+    // free job-table entries (a stack, scheduler-owned)
+    for (int k = lane(); k < S.jcap; k += kWave) S.free_jobs[k] = S.jcap - 1 - k;
+    int nfree_jobs = S.jcap;
     uint64_t head = 0;          // first job not yet admitted or rejected
     uint64_t fetched = 0;       // jobs copied from the host ring into dq
     uint64_t completed = 0;
-    int64_t reserved = 0;
     int running = 0;
+    uint64_t beat = 0, beat_time = wall_clock64();
     // admitted-out-of-order bookkeeping: jobs in [head, head+lookahead) may be admitted
     // early (backfill); admitted[] marks them by ring position.
     for (;;) {
-        // ---- reclaim finished jobs
-        for (int jb = 0; jb < S.jcap; jb++) {
-            JobSlot* js = &S.jobs[jb];
-            int32_t st = 0;
-            if (lane() == 0) st = ld_acquire(&js->state);
-            st = bcast(st);
-            if (st != 2) continue;
+        // ---- reclaim finished jobs: 64 job slots per load, then one pass per finished job
+        for (int base = 0; base < S.jcap; base += kWave) {
+            const int jl = base + lane();
+            const int32_t stl = jl < S.jcap ? ld_relaxed(&S.jobs[jl].state) : 0;
+            uint64_t fin = ballot(stl == 2);
+            if (!fin) continue;
             fence_acquire();
-            for (int i = lane(); i < js->desc.ranks; i += kWave)
-                S.free_slots[nfree + i] = js->slots[i];
-            nfree += js->desc.ranks;
-            reserved -= job_bytes(js->desc);
-            rt_free(js->comm);
-            // publish the result to the host ring
-            const uint64_t pos = completed % (uint64_t)S.qcap;
-            JobResult res = js->res;
-            res.status = res.status == -2 ? -2 : 1;
-            if (lane() == 0) {
-                S.done[pos] = res;
-                __atomic_thread_fence(__ATOMIC_RELEASE);
-                __hip_atomic_store(S.done_count, completed + 1, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            while (fin) {
+                const int jb = base + __builtin_ctzll(fin);
+                fin &= fin - 1;
+                JobSlot* js = &S.jobs[jb];
+                for (int i = lane(); i < js->desc.ranks; i += kWave)
+                    S.free_slots[nfree + i] = js->slots[i];
+                nfree += js->desc.ranks;
+                rt_free(js->comm);
+                buddy_free(S.jobheap, js->arena.base);
+                // publish the result to the host ring
+                const uint64_t pos = completed % (uint64_t)S.qcap;
+                JobResult res = js->res;
+                res.status = res.status == -2 ? -2 : 1;
+                if (lane() == 0) {
+                    S.done[pos] = res;
+                    __atomic_thread_fence(__ATOMIC_RELEASE);
+                    __hip_atomic_store(S.done_count, completed + 1, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                completed++;
+                js->state = 0;
+                S.free_jobs[nfree_jobs++] = jb;
+                running--;
             }
-            completed++;
-            js->state = 0;
-            running--;
         }
         // ---- admit new jobs (FIFO with bounded backfill)
         uint64_t tail = 0;
         if (lane() == 0) tail = sys_load(S.q_tail);
         tail = bcast(tail);
         // one pass over PCIe per new job: copy the descriptors into device memory
-        for (uint64_t qi = fetched + lane(); qi < tail; qi += kWave)
-            S.dq[qi % (uint64_t)S.qcap] = S.q[qi % (uint64_t)S.qcap];
-        fetched = tail;
-        fence_acquire();
+        if (tail != fetched) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's ring writes
+            for (uint64_t qi = fetched + lane(); qi < tail; qi += kWave)
+                S.dq[qi % (uint64_t)S.qcap] = S.q[qi % (uint64_t)S.qcap];
+            fetched = tail;
+        }
         int32_t stop = 0;
-        if (lane() == 0) stop = sys_load32(S.stop);
+        uint64_t hb = 0;
+        if (lane() == 0) {
+            stop = sys_load32(S.stop);
+            hb = sys_load(S.heartbeat);
+        }
         stop = bcast(stop);
+        hb = bcast(hb);
+        // a host that stopped calling submit/poll is gone: drain and leave, so the grid
+        // always ends
+        const uint64_t now = wall_clock64();
+        if (hb != beat) {
+            beat = hb;
+            beat_time = now;
+        } else if (now - beat_time > S.idle_ticks) {
+            stop = 1;
+        }
         for (uint64_t qi = head; qi < tail && qi < head + (uint64_t)S.lookahead; qi++) {
             JobDesc d = S.dq[qi % (uint64_t)S.qcap];
             if (d.id == ~0ull) {  // already admitted by backfill
@@ -299,13 +404,39 @@ __device__ void serve_scheduler(const ServeShared& S) {
                 if (qi == head) head++;
                 continue;
             }
-            if (d.ranks > nfree || reserved + need > S.heap_budget) continue;
-            int jb = -1;
-            for (int k = 0; k < S.jcap; k++)
-                if (S.jobs[k].state == 0) { jb = k; break; }
-            if (jb < 0) break;
+            if (d.ranks > nfree) continue;
+            if (nfree_jobs == 0) break;
+            const int jb = S.free_jobs[nfree_jobs - 1];
+            // admission by real headroom: the job's whole arena must be allocatable now
+            char* ab = buddy_alloc(S.jobheap, (uint64_t)need);
+            if (!ab) {
+                if (running == 0) {
+                    // nothing will free memory: report it as rejected
+                    JobResult res{};
+                    res.id = d.id;
+                    res.status = -1;
+                    res.ranks = d.ranks;
+                    res.bytes = need;
+                    const uint64_t pos = completed % (uint64_t)S.qcap;
+                    if (lane() == 0) {
+                        S.done[pos] = res;
+                        __atomic_thread_fence(__ATOMIC_RELEASE);
+                        __hip_atomic_store(S.done_count, completed + 1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                        S.stats[1]++;
+                    }
+                    completed++;
+                    S.dq[qi % (uint64_t)S.qcap].id = ~0ull;
+                    if (qi == head) head++;
+                }
+                continue;
+            }
             CommImpl* c = comm_alloc(d.ranks);
-            if (!c) break;
+            if (!c) {
+                buddy_free(S.jobheap, ab);
+                break;
+            }
+            nfree_jobs--;
             JobSlot* js = &S.jobs[jb];
             int32_t* list = S.slot_lists + (int64_t)jb * S.max_ranks;
             for (int i = lane(); i < d.ranks; i += kWave) {
@@ -315,11 +446,13 @@ __device__ void serve_scheduler(const ServeShared& S) {
                 c->inv[w] = i;
             }
             nfree -= d.ranks;
-            reserved += need;
             js->desc = d;
             js->comm = c;
             js->slots = list;
             js->remaining = d.ranks;
+            js->arena.base = ab;
+            js->arena.size = (uint64_t)need;
+            js->arena.bump = 0;
             js->res = JobResult{};
             js->res.id = d.id;
             js->res.ranks = d.ranks;

@@ -85,12 +85,6 @@ __device__ __forceinline__ int64_t wave_search_le(const uint64_t* a, int64_t n, 
     return lo;
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int k) {
-    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), k);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // ------------------------------------------------------------------------------ plan
 // One workgroup of four wavefronts per destination row d; wave w owns a contiguous
 // quarter of the sources.  Pass 1 sums the quarter's counts (the waves' totals give each

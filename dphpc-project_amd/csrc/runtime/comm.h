@@ -36,8 +36,10 @@ __device__ bool comm_barrier(CommImpl* c) {
     fence_release();
     uint32_t g = 0, last = 0;
     if (lane() == 0) {
-        g = ld_acquire(&c->gen.value);
-        uint32_t old = fetch_add(&c->arrive.value, 1u);
+        g = ld_relaxed(&c->gen.value);
+        // relaxed arrival: the release fence above and the acquire fence below order
+        // the payloads (one L2 write-back and one invalidate per barrier, not per poll)
+        uint32_t old = fetch_add_relaxed(&c->arrive.value, 1u);
         if (old == (uint32_t)c->size - 1u) {
             st_relaxed(&c->arrive.value, 0u);
             st_release(&c->gen.value, g + 1u);
@@ -50,7 +52,7 @@ __device__ bool comm_barrier(CommImpl* c) {
         Waiter w;
         for (;;) {
             uint32_t cur = 0;
-            if (lane() == 0) cur = ld_acquire(&c->gen.value);
+            if (lane() == 0) cur = ld_relaxed(&c->gen.value);
             cur = bcast(cur);
             if (cur != g) break;
             progress();
@@ -87,7 +89,7 @@ __device__ bool coll_wait_slot(CommImpl* c, int j, uint64_t seq) {
     Waiter w;
     for (;;) {
         uint64_t s = 0;
-        if (lane() == 0) s = ld_acquire(&c->slots[j].seq);
+        if (lane() == 0) s = ld_relaxed(&c->slots[j].seq);
         s = bcast(s);
         if (s >= seq) break;
         progress();
@@ -102,7 +104,7 @@ __device__ bool coll_wait_all(CommImpl* c, uint64_t seq) {
     for (;;) {
         bool ok = true;
         for (int j = lane(); j < c->size; j += kWave)
-            if (ld_acquire(&c->slots[j].seq) < seq) ok = false;
+            if (ld_relaxed(&c->slots[j].seq) < seq) ok = false;
         if (ballot(!ok) == 0) break;
         progress();
         if (!w.keep_going(__LINE__)) return false;

@@ -48,6 +48,12 @@ __device__ __forceinline__ uint32_t readlane(uint32_t v, int l) {
     return (uint32_t)__builtin_amdgcn_readlane((int32_t)v, l);
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int k) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), k);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Wave-wide reductions over 64 lanes (uniform result).
 __device__ __forceinline__ int64_t wave_sum(int64_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

@@ -16,6 +16,9 @@
 //     units) with a 24-bit ABA tag;
 //   * a refill carves a fresh page, links its blocks with all 64 lanes and pushes the
 //     chain with one CAS;
+//   * when the page bump is used up, a refill takes a free block of the smallest
+//     larger page-run class instead and splits it buddy-style (the upper halves go to
+//     the classes in between), so freed large blocks serve later smaller requests;
 //   * a small per-rank cache per class absorbs alloc/free churn without atomics;
 //   * bytes_in_use is the headroom counter that the serving loop admits against.
 #pragma once
@@ -90,6 +93,34 @@ __device__ __forceinline__ uint64_t heap_pop(Heap* h, int c) {
     }
 }
 
+// A run of 2^(cp - kHeapSmallClasses) pages (wave-collective): from the page bump while
+// it lasts (the bump only grows; pages past the end are never handed out), else the
+// first free block of class cp or larger, split down to cp -- each split pushes the
+// upper half to its own class.  Returns the first page, or ~0 when nothing is left.
+__device__ uint64_t heap_take_pages(Heap* h, int cp) {
+    const uint64_t pages = 1ull << (cp - kHeapSmallClasses);
+    uint64_t first = 0;
+    if (lane() == 0) first = fetch_add_relaxed(&h->page_bump, pages);
+    first = bcast(first);
+    if (first + pages <= h->npages) return first;
+    for (int k = cp; k < kHeapClasses; k++) {
+        uint64_t off = kHeapEmpty;
+        if (lane() == 0) off = heap_pop(h, k);
+        off = bcast(off);
+        if (off == kHeapEmpty) continue;
+        const uint64_t page = (off << 4) >> kHeapPageShift;
+        for (int j = k - 1; j >= cp; j--) {
+            const uint64_t upper = page + (1ull << (j - kHeapSmallClasses));
+            h->page_class[upper] = (uint8_t)j;
+            fence_release();
+            const uint64_t uoff = (upper << kHeapPageShift) >> 4;
+            if (lane() == 0) heap_push_chain(h, j, uoff, uoff);
+        }
+        return page;
+    }
+    return ~0ull;
+}
+
 // Wave-collective allocation.  Returns nullptr when the arena is exhausted.
 __device__ void* heap_alloc(Heap* h, HeapCache* cache, uint64_t n);
 __device__ void heap_free(Heap* h, HeapCache* cache, void* p);
@@ -127,16 +158,10 @@ __device__ void* heap_alloc(Heap* h, HeapCache* cache, uint64_t n) {
         heap_account(h, (int64_t)bsz);
         return h->base + (off << 4);
     }
-    // 3. carve fresh pages
-    const uint64_t pages = c < kHeapSmallClasses ? 1 : (bsz >> kHeapPageShift);
-    uint64_t first = 0;
-    if (lane() == 0) first = fetch_add_relaxed(&h->page_bump, pages);
-    first = bcast(first);
-    if (first + pages > h->npages) {
-        // Roll back is impossible without a lock; the pages past the end are simply
-        // never handed out (the bump only ever grows).
-        return nullptr;
-    }
+    // 3. a run of whole pages: fresh from the bump, else split a larger free block
+    const int cp = c < kHeapSmallClasses ? kHeapSmallClasses : c;
+    const uint64_t first = heap_take_pages(h, cp);
+    if (first == ~0ull) return nullptr;
     h->page_class[first] = (uint8_t)c;
     if (lane() == 0) fetch_add_relaxed(&h->refill_count, (int64_t)1);
     const uint64_t page_off = (first << kHeapPageShift) >> 4;  // 16 B units

@@ -593,10 +593,26 @@ __device__ int MPI_Attr_get(MPI_Comm, int, void*, int* flag) {
 __device__ int MPI_Attr_put(MPI_Comm, int, void*) { return MPI_SUCCESS; }
 
 // ---------------------------------------------------------------- device heap API
+// A rank that runs a served job allocates from the job's arena (256-byte granules, one
+// atomic bump shared by the job's ranks); frees inside the arena are no-ops because the
+// scheduler releases the arena in one piece when the job ends.
 __device__ void* __gpu_malloc(size_t size) {
+    JobArena* a = my_state()->arena;
+    if (a) {
+        if (size == 0) return nullptr;
+        const uint64_t n = ((uint64_t)size + 255) & ~255ull;
+        uint64_t off = 0;
+        if (lane() == 0) off = fetch_add_relaxed(&a->bump, n);
+        off = bcast(off);
+        return off + n <= a->size ? a->base + off : nullptr;
+    }
     return heap_alloc(&rt()->heap, &my_state()->cache, size);
 }
-__device__ void __gpu_free(void* p) { heap_free(&rt()->heap, &my_state()->cache, p); }
+__device__ void __gpu_free(void* p) {
+    JobArena* a = my_state()->arena;
+    if (a && (char*)p >= a->base && (char*)p < a->base + a->size) return;
+    heap_free(&rt()->heap, &my_state()->cache, p);
+}
 __device__ void* __gpu_calloc(size_t nmemb, size_t size) {
     uint64_t n = (uint64_t)nmemb * size;
     void* p = __gpu_malloc(n);

@@ -67,6 +67,14 @@ struct Request {
 
 constexpr uint64_t kScratchBytes = 64 * 1024;
 
+// Bump region of one served job: the job's ranks allocate from it, the scheduler frees
+// it in one piece when the job ends.
+struct JobArena {
+    char* base;
+    uint64_t size;
+    uint64_t bump;  // atomic, bytes handed out
+};
+
 struct RankState {
     int32_t world_rank;
     int32_t n_posted;
@@ -81,6 +89,7 @@ struct RankState {
     int32_t initialized;
     int32_t finalized;
     HeapCache cache;
+    JobArena* arena;      // non-null while the rank runs a served job
 };
 
 struct GroupImpl {

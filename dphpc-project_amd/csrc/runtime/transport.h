@@ -234,8 +234,10 @@ __device__ int progress() {
     int consumed = 0;
     for (uint32_t guard = 0; guard < N; guard++) {
         Envelope* e = &ring[h % N];
+        // poll with a plain coherent load; the acquire fence (an L2 invalidate on gfx950)
+        // only once an envelope is there
         uint32_t seq = 0;
-        if (lane() == 0) seq = ld_acquire(&e->seq);
+        if (lane() == 0) seq = ld_relaxed(&e->seq);
         seq = bcast(seq);
         if (seq != (uint32_t)(h + 1)) break;
         fence_acquire();
@@ -272,12 +274,13 @@ __device__ bool push_envelope(int dest, u32x4 v) {
     Waiter w;
     for (;;) {
         uint32_t s = 0;
-        if (lane() == 0) s = ld_acquire(&e->seq);
+        if (lane() == 0) s = ld_relaxed(&e->seq);
         s = bcast(s);
         if (s == (uint32_t)t) break;
         progress();  // drain our own ring so two full rings cannot deadlock
         if (!w.keep_going(__LINE__)) return false;
     }
+    fence_acquire();  // the consumer's reads of this slot happen before our writes
     // lane 0 must not overwrite seq with the plain store; it writes words 1..3 only
     if (lane() >= 1 && lane() < 8) ((u32x4*)e)[lane()] = v;
     if (lane() == 0) {
@@ -384,7 +387,7 @@ __device__ bool test_request(Request* r) {
     if (r->state == kReqDone) return true;
     if (r->kind == kReqSend) {
         uint32_t d = 0;
-        if (lane() == 0) d = ld_acquire(&r->done);
+        if (lane() == 0) d = ld_relaxed(&r->done);
         d = bcast(d);
         if (d) {
             fence_acquire();

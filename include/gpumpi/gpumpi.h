@@ -187,7 +187,10 @@ int gpumpi_npb_is_run(char cls, int32_t P, int64_t* passed, double* exchange_ms)
  *           bytes in .checksum.
  * gpumpi_serve_*: a persistent rank grid of cfg->num_ranks wavefront slots; slot 0 admits
  * jobs in arrival order (bounded backfill) when enough slots are idle and the job's
- * heap need fits in heap_fraction of the device heap; the others run jobs.
+ * heap need fits in heap_fraction of the device heap; the others run jobs.  The grid
+ * holds the GPU until gpumpi_serve_stop, or until cfg->timeout_s passes without a
+ * gpumpi_serve_submit/poll call (a host that went away): then it finishes the admitted
+ * jobs and exits by itself.
  * gpumpi_run_job: the plain path -- one rank-grid launch per job.
  * ------------------------------------------------------------------------------------ */
 typedef struct gpumpi_job {
