@@ -428,7 +428,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args)
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "a2av_step_uniform_512x2^18_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "a2av_copy_kernel_uniform_p512_k262144_traffic.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
