@@ -5,8 +5,8 @@ Workload (the bucket-sort exchange): P = 512 ranks, 2^18 uint32 keys per rank (2
 keys, 512 MiB) uniform in [0, 2^23), bucketed NPB-style into 1024 buckets (NPB IS
 rank(): histogram, splitters, stable bucketing).  One timed step = the batched exchange:
 Alltoall of the counts + receive displacements + the skewed Alltoallv of the keys, as
-three kernels (plan, scan, copy) replayed from one hipGraph.  Inputs are resident in HBM
-before the timed region.
+three kernels (plan, scan, copy), replayed from a hipGraph that holds 10 steps (one graph
+launch per 10 steps).  Inputs are resident in HBM before the timed region.
 
 value  = algorithmic bytes / time = 2 x payload bytes (read once + written once) per
          step / seconds per step, in GB/s.
@@ -317,8 +317,8 @@ def serve_latency(pkg, torch, args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--ranks", type=int, default=512)
     ap.add_argument("--keys-log2", type=int, default=18)
     ap.add_argument("--plain-steps", type=int, default=5)
@@ -331,7 +331,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 2.0])
-    ap.add_argument("--tune", type=str, default="", help="block_threads,blocks_per_cu,lines,nt")
+    ap.add_argument("--tune", type=str, default="",
+                    help="block_threads,blocks_per_cu,lines_per_lane,nontemporal[,chunk_rounds]")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -357,12 +358,16 @@ def main():
     if not args.no_autotune and tune is None:
         chosen, us = plan.autotune(stream)
         log(f"autotune: {chosen} best {us:.1f} us")
-    for _ in range(max(1, args.warmup)):
-        plan.graph_replay(1, stream)
+    # steps are replayed from a graph of g captured steps (one graph launch per g steps);
+    # one checked step first, then the warm-up right before the timed steps
+    g = 10 if args.steps % 10 == 0 and args.warmup % 10 == 0 else 1
+    plan.graph_replay(g, stream)
     torch.cuda.synchronize()
     ex.verify(torch)
     assert plan.payload_bytes() == ex.payload
-    ms = time_steps(torch, lambda: plan.graph_replay(1, stream), args.steps)
+    for _ in range(max(1, args.warmup // g)):
+        plan.graph_replay(g, stream)
+    ms = time_steps(torch, lambda: plan.graph_replay(g, stream), args.steps // g) / g
     algo = 2 * ex.payload
     gbs = algo / (ms * 1e-3) / 1e9
     log(f"batched: {ms*1e3:.1f} us/step, {gbs:.1f} GB/s ({100*gbs/HBM_PEAK_GBS:.1f}% of HBM peak), "
@@ -404,12 +409,14 @@ def main():
         del plan
         ez = Exchange(pkg, torch, P, n, "zipf")
         zp = ez.plan(pkg, (tune_used["block_threads"], tune_used["blocks_per_cu"],
-                           tune_used["lines_per_lane"], tune_used["nontemporal"]))
-        for _ in range(max(1, args.warmup)):
-            zp.graph_replay(1, stream)
+                           tune_used["lines_per_lane"], tune_used["nontemporal"],
+                           tune_used["chunk_rounds"]))
+        zp.graph_replay(g, stream)
         torch.cuda.synchronize()
         ez.verify(torch)
-        zms = time_steps(torch, lambda: zp.graph_replay(1, stream), args.steps)
+        for _ in range(max(1, args.warmup // g)):
+            zp.graph_replay(g, stream)
+        zms = time_steps(torch, lambda: zp.graph_replay(g, stream), args.steps // g) / g
         zipf = {"ms_per_step": round(zms, 5), "GBps": round(2 * ez.payload / (zms * 1e-3) / 1e9, 2),
                 "max_recv_bytes": int(ez.rtot.max()) * 4, "payload_bytes": ez.payload}
         log(f"zipf: {zms*1e3:.1f} us/step, {zipf['GBps']} GB/s, heaviest receiver "
@@ -450,9 +457,9 @@ def main():
         "data": "synthetic (Philox uniform keys, NPB-style buckets)",
         "config": {"workload": f"bucket-sort exchange: {P} ranks x 2^{args.keys_log2} uint32 keys, "
                                "uniform [0,2^23), 1024 buckets; one step = Alltoall(counts) + "
-                               "Alltoallv(keys) as plan/scan/copy kernels replayed as one hipGraph",
+                               "Alltoallv(keys) as plan/scan/copy kernels, replayed from a hipGraph of 10 steps",
                    "ranks": P, "keys_per_rank": n, "payload_bytes": ex.payload,
-                   "tune": tune_used,
+                   "tune": tune_used, "graph_steps": g,
                    "kernel_us": {k: round(v, 2) for k, v in kern.items()}},
         "roofline": {"bound": "hbm", "kernel": "a2av_copy_kernel",
                      "achieved": round(copy_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

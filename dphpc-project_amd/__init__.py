@@ -61,6 +61,7 @@ class A2avTune(ctypes.Structure):
     _fields_ = [
         ("block_threads", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("lines_per_lane", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
+        ("chunk_rounds", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -659,7 +659,6 @@ struct gpumpi_a2av_plan {
 
 static const void* a2av_copy_for(int U, bool nt) {
     switch (U) {
-    case 1: return nt ? (const void*)&a2av_copy_kernel<1, true> : (const void*)&a2av_copy_kernel<1, false>;
     case 2: return nt ? (const void*)&a2av_copy_kernel<2, true> : (const void*)&a2av_copy_kernel<2, false>;
     case 4: return nt ? (const void*)&a2av_copy_kernel<4, true> : (const void*)&a2av_copy_kernel<4, false>;
     default: return nt ? (const void*)&a2av_copy_kernel<8, true> : (const void*)&a2av_copy_kernel<8, false>;
@@ -681,11 +680,14 @@ static void a2av_size_grid(gpumpi_a2av_plan* pl) {
 
 static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
     const int U = t.lines_per_lane;
-    if (U != 1 && U != 2 && U != 4 && U != 8) return GPUMPI_E_ARG;
+    if (U != 2 && U != 4 && U != 8) return GPUMPI_E_ARG;
+    const int R = t.chunk_rounds <= 0 ? 1 : t.chunk_rounds;
+    if (R != 1 && R != 2 && R != 4 && R != 8) return GPUMPI_E_ARG;
     if (t.block_threads != kA2avCopyThreads || t.blocks_per_cu < 0) return GPUMPI_E_ARG;
     pl->tune = t;
     pl->tune.nontemporal = t.nontemporal > 0 ? 1 : 0;
-    pl->dev.chunk_lines = (uint32_t)(kA2avCopyThreads * U);
+    pl->tune.chunk_rounds = R;
+    pl->dev.chunk_lines = (uint32_t)(kA2avCopyThreads * U * R);
     a2av_size_grid(pl);
     return GPUMPI_OK;
 }
@@ -761,12 +763,13 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
     d.chunk_off = d.row_bytes + P;
     d.total_bytes = (int64_t*)(d.chunk_off + P + 1);
     for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
-    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1};
+    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4};
     if (tune) {
         if (tune->block_threads > 0) t.block_threads = tune->block_threads;
         if (tune->blocks_per_cu >= 0) t.blocks_per_cu = tune->blocks_per_cu;
         if (tune->lines_per_lane > 0) t.lines_per_lane = tune->lines_per_lane;
         if (tune->nontemporal >= 0) t.nontemporal = tune->nontemporal;
+        if (tune->chunk_rounds > 0) t.chunk_rounds = tune->chunk_rounds;
     }
     int rc = a2av_configure(pl, t);
     if (rc != GPUMPI_OK) {
@@ -868,8 +871,9 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
     int64_t bytes = 0;
     if (a2av_read_bytes(pl, &bytes) != GPUMPI_OK) return GPUMPI_E_HIP;
     pl->est_bytes = bytes;
-    const int bpcs[] = {0, 64, 16, 4};
-    const int us_[] = {1, 2, 4, 8};
+    const int bpcs[] = {0, 16, 4};
+    const int us_[] = {2, 4, 8};
+    const int rs[] = {1, 2, 4};
     const int nts[] = {1, 0};
     gpumpi_a2av_tune best = pl->tune;
     double best_t = 1e30;
@@ -878,8 +882,9 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
     hipEventCreate(&e1);
     for (int bpc : bpcs)
         for (int U : us_)
+            for (int R : rs)
             for (int nt : nts) {
-                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt};
+                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt, R};
                 if (a2av_configure(pl, t) != GPUMPI_OK) continue;
                 a2av_launch(pl, s, false);  // warm-up
                 const int reps = 5;

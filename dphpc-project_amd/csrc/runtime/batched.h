@@ -87,8 +87,11 @@ __device__ __forceinline__ int64_t wave_search_le(const uint64_t* a, int64_t n, 
 
 // ------------------------------------------------------------------------------ plan
 // One workgroup of four wavefronts per destination row d; wave w owns a contiguous
-// quarter of the sources.  Pass 1 sums the quarter's counts (the waves' totals give each
-// quarter's starting displacement), pass 2 scans and writes.
+// quarter of the sources.  The first 256 sources of a quarter (all of them up to P = 1024)
+// are loaded once -- counts, displacements and send buffers in one round of loads -- and
+// kept in registers: the quarters' count totals meet in LDS to give each quarter's
+// starting displacement, then each wave scans and writes.  Sources past 256 per quarter
+// are summed in a first pass and reloaded in the second.
 __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
     __shared__ int64_t s_tot[4];
     const int P = p.P;
@@ -98,13 +101,21 @@ __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
     const int Q = (((P + 3) / 4) + kWave - 1) / kWave * kWave;
     const int sb = w * Q;
     const int se = sb + Q < P ? sb + Q : P;
+    int64_t c[4], sd[4];
+    const char* sbuf[4];
     int64_t part = 0;
-    for (int s0 = sb; s0 < se; s0 += 4 * kWave) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int s = s0 + q * kWave + l;
-            part += s < se ? p.scounts[(int64_t)s * P + d] : 0;
-        }
+    for (int q = 0; q < 4; q++) {
+        const int s = sb + q * kWave + l;
+        const bool valid = s < se;
+        c[q] = valid ? p.scounts[(int64_t)s * P + d] : 0;
+        sd[q] = valid ? p.sdispls[(int64_t)s * P + d] : 0;
+        sbuf[q] = valid ? p.sendbufs[s] : nullptr;
+        part += c[q];
+    }
+    for (int s0 = sb + 4 * kWave; s0 < se; s0 += kWave) {
+        const int s = s0 + l;
+        part += s < se ? p.scounts[(int64_t)s * P + d] : 0;
     }
     part = wave_sum(part);
     if (l == 0) s_tot[w] = part;
@@ -114,32 +125,30 @@ __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
         carry += k < w ? s_tot[k] : 0;
         row += s_tot[k];
     }
-    for (int s0 = sb; s0 < se; s0 += 4 * kWave) {
-        // the strided column loads of up to four 64-source chunks go out before any scan
-        int64_t c[4], sd[4];
-        const char* sbuf[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int s = s0 + q * kWave + l;
+    for (int s0 = sb; s0 < se; s0 += kWave) {
+        const int q = (s0 - sb) / kWave;
+        const int s = s0 + l;
+        int64_t cq, sdq;
+        const char* sbq;
+        if (q < 4) {
+            cq = q == 0 ? c[0] : q == 1 ? c[1] : q == 2 ? c[2] : c[3];
+            sdq = q == 0 ? sd[0] : q == 1 ? sd[1] : q == 2 ? sd[2] : sd[3];
+            sbq = q == 0 ? sbuf[0] : q == 1 ? sbuf[1] : q == 2 ? sbuf[2] : sbuf[3];
+        } else {
             const bool valid = s < se;
-            c[q] = valid ? p.scounts[(int64_t)s * P + d] : 0;
-            sd[q] = valid ? p.sdispls[(int64_t)s * P + d] : 0;
-            sbuf[q] = valid ? p.sendbufs[s] : nullptr;
+            cq = valid ? p.scounts[(int64_t)s * P + d] : 0;
+            sdq = valid ? p.sdispls[(int64_t)s * P + d] : 0;
+            sbq = valid ? p.sendbufs[s] : nullptr;
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (s0 + q * kWave >= se) break;
-            const int s = s0 + q * kWave + l;
-            int64_t tot;
-            const int64_t rd = wave_excl_scan(c[q], &tot) + carry;
-            carry += tot;
-            if (s < se) {
-                const int64_t o = d * P + s;
-                p.rcounts[o] = (int32_t)c[q];
-                p.rdispls[o] = (int32_t)rd;
-                p.seg_end[o] = (uint64_t)(rd + c[q]) * (uint64_t)p.elem;
-                p.seg_src[o] = (uint64_t)(uintptr_t)(sbuf[q] + sd[q] * p.elem);
-            }
+        int64_t tot;
+        const int64_t rd = wave_excl_scan(cq, &tot) + carry;
+        carry += tot;
+        if (s < se) {
+            const int64_t o = d * P + s;
+            p.rcounts[o] = (int32_t)cq;
+            p.rdispls[o] = (int32_t)rd;
+            p.seg_end[o] = (uint64_t)(rd + cq) * (uint64_t)p.elem;
+            p.seg_src[o] = (uint64_t)(uintptr_t)(sbq + sdq * p.elem);
         }
     }
     if (threadIdx.x == 0) p.row_bytes[d] = (uint64_t)row * (uint64_t)p.elem;

@@ -124,9 +124,10 @@ typedef struct gpumpi_a2av_tune {
     int32_t block_threads;      /* copy workgroup size: 256 (0 = default) */
     int32_t blocks_per_cu;      /* copy workgroups per CU at most; 0 = one per chunk,
                                    -1 = default */
-    int32_t lines_per_lane;     /* 16-byte lines per thread and chunk: 1, 2, 4, 8
-                                   (chunk = 256 * lines); 0 = default */
+    int32_t lines_per_lane;     /* 16-byte lines per thread and round: 2, 4, 8; 0 = default */
     int32_t nontemporal;        /* 1 = non-temporal stores, 0 = cached, -1 = default */
+    int32_t chunk_rounds;       /* rounds of block_threads * lines per chunk: 1, 2, 4, 8;
+                                   0 = default */
 } gpumpi_a2av_tune;
 
 int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,

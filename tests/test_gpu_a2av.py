@@ -82,7 +82,7 @@ def test_a2av_random(P, elem, maxc, pkg, oracle, torch_gpu):
         assert np.array_equal(got[d], exp[d][:got[d].size]), f"rank {d} payload differs"
 
 
-@pytest.mark.parametrize("tune", [(256, 0, 1, 0), (256, 16, 2, 1), (256, 1, 8, 1), (256, 4, 4, 0)])
+@pytest.mark.parametrize("tune", [(256, 0, 2, 0, 1), (256, 16, 2, 1, 2), (256, 1, 8, 1, 1), (256, 4, 4, 0, 4), (256, 0, 4, 1, 8)])
 def test_a2av_launch_shapes(tune, pkg, oracle, torch_gpu):
     print(f"[gpu] a2av launch shape {tune}", flush=True)
     rng = np.random.default_rng(11)
