@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 2.0])
     ap.add_argument("--tune", type=str, default="",
-                    help="block_threads,blocks_per_cu,lines_per_lane,nontemporal[,chunk_rounds]")
+                    help="block_threads,blocks_per_cu,lines_per_lane,nontemporal[,chunk_rounds[,defer_straddle]]")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -410,7 +410,10 @@ def main():
         ez = Exchange(pkg, torch, P, n, "zipf")
         zp = ez.plan(pkg, (tune_used["block_threads"], tune_used["blocks_per_cu"],
                            tune_used["lines_per_lane"], tune_used["nontemporal"],
-                           tune_used["chunk_rounds"]))
+                           tune_used["chunk_rounds"], tune_used["defer_straddle"]))
+        if not args.no_autotune and tune is None:
+            zchosen, zus = zp.autotune(stream)
+            log(f"zipf autotune: {zchosen} best {zus:.1f} us")
         zp.graph_replay(g, stream)
         torch.cuda.synchronize()
         ez.verify(torch)
@@ -418,7 +421,8 @@ def main():
             zp.graph_replay(g, stream)
         zms = time_steps(torch, lambda: zp.graph_replay(g, stream), args.steps // g) / g
         zipf = {"ms_per_step": round(zms, 5), "GBps": round(2 * ez.payload / (zms * 1e-3) / 1e9, 2),
-                "max_recv_bytes": int(ez.rtot.max()) * 4, "payload_bytes": ez.payload}
+                "max_recv_bytes": int(ez.rtot.max()) * 4, "payload_bytes": ez.payload,
+                "tune": zp.tune()}
         log(f"zipf: {zms*1e3:.1f} us/step, {zipf['GBps']} GB/s, heaviest receiver "
             f"{zipf['max_recv_bytes']/2**20:.1f} MiB")
         zp.close()

@@ -61,7 +61,7 @@ class A2avTune(ctypes.Structure):
     _fields_ = [
         ("block_threads", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("lines_per_lane", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
-        ("chunk_rounds", ctypes.c_int32),
+        ("chunk_rounds", ctypes.c_int32), ("defer_straddle", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -344,7 +344,7 @@ class A2avPlan:
     def __init__(self, P, elem_bytes, sendbufs_ptr, scounts_ptr, sdispls_ptr, recvbufs_ptr,
                  rcounts_ptr, rdispls_ptr, tune=None):
         self.lib = load_library()
-        t = A2avTune(0, -1, 0, -1) if tune is None else A2avTune(*tune)
+        t = A2avTune(0, -1, 0, -1, 0, -1) if tune is None else A2avTune(*tune)
         h = _P()
         rc = self.lib.gpumpi_a2av_plan_create(P, elem_bytes, sendbufs_ptr, scounts_ptr,
                                               sdispls_ptr, recvbufs_ptr, rcounts_ptr,

@@ -657,12 +657,17 @@ struct gpumpi_a2av_plan {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-static const void* a2av_copy_for(int U, bool nt) {
+template <bool DEFER>
+static const void* a2av_copy_for_mode(int U, bool nt) {
     switch (U) {
-    case 2: return nt ? (const void*)&a2av_copy_kernel<2, true> : (const void*)&a2av_copy_kernel<2, false>;
-    case 4: return nt ? (const void*)&a2av_copy_kernel<4, true> : (const void*)&a2av_copy_kernel<4, false>;
-    default: return nt ? (const void*)&a2av_copy_kernel<8, true> : (const void*)&a2av_copy_kernel<8, false>;
+    case 2: return nt ? (const void*)&a2av_copy_kernel<2, true, DEFER> : (const void*)&a2av_copy_kernel<2, false, DEFER>;
+    case 4: return nt ? (const void*)&a2av_copy_kernel<4, true, DEFER> : (const void*)&a2av_copy_kernel<4, false, DEFER>;
+    default: return nt ? (const void*)&a2av_copy_kernel<8, true, DEFER> : (const void*)&a2av_copy_kernel<8, false, DEFER>;
     }
+}
+
+static const void* a2av_copy_for(int U, bool nt, bool defer) {
+    return defer ? a2av_copy_for_mode<true>(U, nt) : a2av_copy_for_mode<false>(U, nt);
 }
 
 // Copy grid: one workgroup per chunk of the last measured step (P extra for the partial
@@ -675,7 +680,12 @@ static void a2av_size_grid(gpumpi_a2av_plan* pl) {
         g = (int64_t)pl->tune.blocks_per_cu * pl->cus;
     if (g < 1) g = 1;
     if (g > (1 << 22)) g = 1 << 22;
-    pl->grid = (int)g;
+    // deferred mode: one thread per segment in leading blocks, at most four per CU
+    const uint64_t PP = (uint64_t)pl->dev.P * (uint64_t)pl->dev.P;
+    const uint64_t sblocks = (PP + kA2avCopyThreads - 1) / kA2avCopyThreads;
+    const uint64_t scap = 4ull * (uint64_t)(pl->cus > 0 ? pl->cus : 1);
+    pl->dev.straddle_blocks = pl->tune.defer_straddle > 0 ? (int32_t)(sblocks < scap ? sblocks : scap) : 0;
+    pl->grid = (int)g + pl->dev.straddle_blocks;
 }
 
 static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
@@ -687,6 +697,7 @@ static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
     pl->tune = t;
     pl->tune.nontemporal = t.nontemporal > 0 ? 1 : 0;
     pl->tune.chunk_rounds = R;
+    pl->tune.defer_straddle = t.defer_straddle > 0 ? 1 : 0;
     pl->dev.chunk_lines = (uint32_t)(kA2avCopyThreads * U * R);
     a2av_size_grid(pl);
     return GPUMPI_OK;
@@ -702,7 +713,9 @@ static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s, bool timed) {
     if (hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
     if (timed) hipEventRecord(pl->ev[2], s);
-    if (hipLaunchKernel(a2av_copy_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0), dim3(pl->grid),
+    if (hipLaunchKernel(a2av_copy_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0,
+                                      pl->tune.defer_straddle > 0),
+                        dim3(pl->grid),
                         dim3(kA2avCopyThreads), kargs, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
     if (timed) hipEventRecord(pl->ev[3], s);
@@ -763,13 +776,14 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
     d.chunk_off = d.row_bytes + P;
     d.total_bytes = (int64_t*)(d.chunk_off + P + 1);
     for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
-    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4};
+    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4, 0};
     if (tune) {
         if (tune->block_threads > 0) t.block_threads = tune->block_threads;
         if (tune->blocks_per_cu >= 0) t.blocks_per_cu = tune->blocks_per_cu;
         if (tune->lines_per_lane > 0) t.lines_per_lane = tune->lines_per_lane;
         if (tune->nontemporal >= 0) t.nontemporal = tune->nontemporal;
         if (tune->chunk_rounds > 0) t.chunk_rounds = tune->chunk_rounds;
+        if (tune->defer_straddle >= 0) t.defer_straddle = tune->defer_straddle;
     }
     int rc = a2av_configure(pl, t);
     if (rc != GPUMPI_OK) {
@@ -875,16 +889,18 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
     const int us_[] = {2, 4, 8};
     const int rs[] = {1, 2, 4};
     const int nts[] = {1, 0};
+    const int defers[] = {0, 1};
     gpumpi_a2av_tune best = pl->tune;
     double best_t = 1e30;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
+    for (int defer : defers)
     for (int bpc : bpcs)
         for (int U : us_)
             for (int R : rs)
             for (int nt : nts) {
-                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt, R};
+                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt, R, defer};
                 if (a2av_configure(pl, t) != GPUMPI_OK) continue;
                 a2av_launch(pl, s, false);  // warm-up
                 const int reps = 5;

@@ -22,9 +22,12 @@
 //         (it only moves forward) and choosing each lane's segment among the cursor's next
 //         four with compares.  A line inside one segment is one unaligned 16-byte load
 //         and one aligned 16-byte store; a line that straddles segments is four dword
-//         loads (issued with the other lines' loads) and one 16-byte store.  Lines at a row's ends, and dwords that are split
-//         or unaligned at the source, go byte by byte; bytes outside the row are never
-//         written.
+//         loads (issued with the other lines' loads) and one 16-byte store -- or, in
+//         deferred mode (a tuning choice that pays when segments are short, as in skewed
+//         key distributions), left to one thread per segment run by the grid's first
+//         blocks, concurrently with the streaming blocks.  Lines at a row's ends, and
+//         dwords that are split or unaligned at the source, go byte by byte; bytes outside
+//         the row are never written.
 //
 // Why not one persistent kernel with a grid barrier: on gfx950 an agent-scope release or
 // acquire writes back or invalidates the XCD's L2 (buffer_wbl2 / buffer_inv sc1); a grid
@@ -47,7 +50,7 @@ struct A2avDev {
     int32_t P;
     int32_t elem;
     uint32_t chunk_lines;   // destination lines per chunk = kA2avCopyThreads * U
-    int32_t pad;
+    int32_t straddle_blocks;  // deferred mode: leading copy blocks running the straddle threads
     char* const* sendbufs;
     const int32_t* scounts;
     const int32_t* sdispls;
@@ -194,6 +197,110 @@ __global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
     }
 }
 
+// ------------------------------------------------------------------------------ straddle
+// Deferred mode: lines inside a row that hold bytes of two or more segments.  One thread
+// per segment of every row; a segment whose end falls strictly inside a 16-byte line, and
+// that holds that line's first byte, owns the line: it assembles it per dword from the
+// segment holding each dword (per byte where a dword is split or unaligned at its source)
+// and stores it once.  Every such line has exactly one owner.  Lines across a row's ends
+// stay with the copy loop.  The copy grid's first straddle_blocks blocks run these
+// threads: their scattered, latency-bound loads overlap the other blocks' streaming.
+__device__ void a2av_straddle_line(const A2avDev& p, uint64_t gid) {
+    const int P = p.P;
+    if (gid >= (uint64_t)P * (uint64_t)P) return;
+    const int64_t d = (int64_t)(gid / (uint64_t)P);
+    const int s = (int)(gid - (uint64_t)d * P);
+    const uint64_t* se = p.seg_end + d * P;
+    // one round of independent loads: this segment, the next one, the row
+    const int s1 = s + 1 < P ? s + 1 : s;
+    const int64_t prev = (int64_t)se[s > 0 ? s - 1 : 0];
+    const int64_t en = (int64_t)se[s];
+    const int64_t en1 = (int64_t)se[s1];
+    const uint64_t src0 = p.seg_src[d * P + s];
+    const uint64_t src1 = p.seg_src[d * P + s1];
+    const uint64_t rbase = (uint64_t)(uintptr_t)p.recvbufs[d];
+    const int64_t rbytes = (int64_t)p.row_bytes[d];
+    const int64_t st = s > 0 ? prev : 0;
+    if (en <= st) return;  // empty
+    const uint64_t le = rbase + (uint64_t)en;  // one past the segment's last byte
+    if ((le & 15) == 0) return;                // ends with its line
+    const uint64_t la = (le >> 4) << 4;        // the line holding the segment's last byte
+    const int64_t off = (int64_t)la - (int64_t)rbase;
+    if (st > off || off < 0 || off + 16 > rbytes) return;
+    // the next non-empty segment
+    int kb = s + 1;
+    int64_t eb = en1;
+    uint64_t srcb = src1;
+    if (en1 <= en) {
+        while ((int64_t)se[kb] <= en) kb++;  // ends before rbytes >= off + 16 > en
+        eb = (int64_t)se[kb];
+        srcb = p.seg_src[d * P + kb];
+    }
+    const uint64_t a0 = src0 + (uint64_t)(off - st);  // source of the line's byte 0 in s
+    const uint64_t a1 = srcb - (uint64_t)en;          // + row offset, in segment kb
+    if (eb >= off + 16) {
+        // two segments: four loads in flight
+        uint64_t addr[4];
+        bool whole[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t q = off + 4 * i;
+            const bool first = q < en;
+            addr[i] = first ? a0 + 4 * i : a1 + (uint64_t)q;
+            whole[i] = (!first || q + 4 <= en) && (addr[i] & 3) == 0;
+        }
+        uint32_t v[4];
+        // a dword split between the segments, or unaligned at its source, is fetched per
+        // byte; its aligned-down dword holds a byte of the source and is a safe read
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            v[i] = *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)(addr[i] & ~(uint64_t)3);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (whole[i]) continue;
+            uint32_t x = 0;
+            for (int j = 0; j < 4; j++) {
+                const int64_t b = off + 4 * i + j;
+                const uint64_t ab = b < en ? a0 + 4 * i + j : a1 + (uint64_t)b;
+                x |= (uint32_t)*(const GPUMPI_GLOBAL uint8_t*)(uintptr_t)ab << (8 * j);
+            }
+            v[i] = x;
+        }
+        *(GPUMPI_GLOBAL u32x4*)(uintptr_t)la = u32x4{v[0], v[1], v[2], v[3]};
+        return;
+    }
+    // three or more segments in the line: walk them
+    int k = s;  // holds byte `off`; the walk only moves forward over the line
+    uint64_t src[4];
+    int whole = 0;
+    for (int i = 0; i < 4; i++) {
+        const int64_t q = off + 4 * i;
+        while (q >= (int64_t)se[k]) k++;
+        const int64_t sk = k > 0 ? (int64_t)se[k - 1] : 0;
+        src[i] = p.seg_src[d * P + k] + (uint64_t)(q - sk);
+        if (q + 4 <= (int64_t)se[k] && (src[i] & 3) == 0) whole |= 1 << i;
+    }
+    uint32_t v[4];
+    k = s;
+    for (int i = 0; i < 4; i++) {
+        if (whole & (1 << i)) {
+            v[i] = *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)src[i];
+            continue;
+        }
+        uint32_t x = 0;
+        for (int j = 0; j < 4; j++) {
+            const int64_t b = off + 4 * i + j;
+            while (b >= (int64_t)se[k]) k++;
+            const int64_t sb = k > 0 ? (int64_t)se[k - 1] : 0;
+            const uint8_t y = *(const GPUMPI_GLOBAL uint8_t*)(uintptr_t)(p.seg_src[d * P + k] +
+                                                                        (uint64_t)(b - sb));
+            x |= (uint32_t)y << (8 * j);
+        }
+        v[i] = x;
+    }
+    *(GPUMPI_GLOBAL u32x4*)(uintptr_t)la = u32x4{v[0], v[1], v[2], v[3]};
+}
+
 // ------------------------------------------------------------------------------ copy
 // One segment seen from its row: bytes [st, en) of the row; the source of the row byte at
 // offset q is (rbase + q) + D.
@@ -248,10 +355,15 @@ __device__ __forceinline__ A2avSegView a2av_seg_at(const A2avDev& p, int64_t d, 
 
 // A destination line that straddles segments or the row's ends: assembled per dword
 // (per byte where a dword is split or its source unaligned); only row bytes are written.
+// In deferred mode the lines inside the row are the straddle threads'.
+template <bool DEFER>
 __device__ void a2av_boundary_line(const A2avDev& p, int64_t d, uint64_t rbase, int64_t rbytes,
                                    const A2avWin& w, uint64_t line) {
     const uint64_t la = line << 4;
     const int64_t off = (int64_t)la - (int64_t)rbase;
+    if constexpr (DEFER) {
+        if (off >= 0 && off + 16 <= rbytes) return;
+    }
     uint32_t v[4] = {0, 0, 0, 0};
     int mask = 0;
     for (int i = 0; i < 4; i++) {
@@ -281,7 +393,7 @@ __device__ void a2av_boundary_line(const A2avDev& p, int64_t d, uint64_t rbase, 
     }
 }
 
-template <int U, bool NT>
+template <int U, bool NT, bool DEFER>
 __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) {
     __shared__ int64_t s_st[kA2avWindow];
     __shared__ int64_t s_en[kA2avWindow];
@@ -291,8 +403,19 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
     const int tid = threadIdx.x;
     const int l = lane();
     const uint64_t C = p.chunk_lines;
+    uint32_t sb = 0;
+    if constexpr (DEFER) {
+        sb = (uint32_t)p.straddle_blocks;
+        if (blockIdx.x < sb) {  // block-uniform
+            const uint64_t nseg = (uint64_t)P * (uint64_t)P;
+            for (uint64_t g = (uint64_t)blockIdx.x * kA2avCopyThreads + tid; g < nseg;
+                 g += (uint64_t)sb * kA2avCopyThreads)
+                a2av_straddle_line(p, g);
+            return;
+        }
+    }
     const uint64_t total = p.chunk_off[P];
-    for (uint64_t ch = blockIdx.x; ch < total; ch += gridDim.x) {
+    for (uint64_t ch = blockIdx.x - sb; ch < total; ch += gridDim.x - sb) {
         // ---- the chunk's row and line range
         if (tid < kWave) {
             const int64_t d = wave_search_le(p.chunk_off, P, ch);
@@ -412,6 +535,7 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                         kind[u] = 1;
                         continue;
                     }
+                    if constexpr (DEFER) continue;
                     // the line straddles segments: one dword from the segment holding it
                     uint64_t a[4];
                     bool ok = true;
@@ -438,7 +562,7 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (kind[u] == 2)
-                        a2av_boundary_line(p, d, rbase, rbytes, w, wb + (uint64_t)u * kWave + l);
+                        a2av_boundary_line<DEFER>(p, d, rbase, rbytes, w, wb + (uint64_t)u * kWave + l);
             }
             __syncthreads();
             pos = pend;

@@ -128,6 +128,9 @@ typedef struct gpumpi_a2av_tune {
     int32_t nontemporal;        /* 1 = non-temporal stores, 0 = cached, -1 = default */
     int32_t chunk_rounds;       /* rounds of block_threads * lines per chunk: 1, 2, 4, 8;
                                    0 = default */
+    int32_t defer_straddle;     /* 1 = lines holding several segments are assembled by one
+                                   thread per segment in leading copy blocks (pays for short
+                                   segments), 0 = by the copy loop, -1 = default (0) */
 } gpumpi_a2av_tune;
 
 int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,

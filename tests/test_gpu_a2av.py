@@ -67,14 +67,16 @@ def random_case(rng, P, elem, maxc, permute=True):
     return send, counts, sdispls
 
 
+@pytest.mark.parametrize("defer", [0, 1])
 @pytest.mark.parametrize("P,elem,maxc", [(1, 4, 100), (2, 1, 50), (7, 1, 333), (16, 4, 300),
-                                         (16, 8, 120), (64, 4, 64), (33, 1, 2000)])
-def test_a2av_random(P, elem, maxc, pkg, oracle, torch_gpu):
-    print(f"[gpu] a2av random P={P} elem={elem} maxc={maxc}", flush=True)
+                                         (16, 8, 120), (64, 4, 64), (33, 1, 2000), (130, 1, 3)])
+def test_a2av_random(P, elem, maxc, defer, pkg, oracle, torch_gpu):
+    print(f"[gpu] a2av random P={P} elem={elem} maxc={maxc} defer={defer}", flush=True)
     rng = np.random.default_rng(P * 1000 + elem)
     send, counts, sdispls = random_case(rng, P, elem, maxc)
     offsets = [int(rng.integers(0, 16)) for _ in range(P)]
-    got, rc, rd, nbytes = run_exchange(pkg, torch_gpu, send, counts, sdispls, elem, offsets=offsets)
+    got, rc, rd, nbytes = run_exchange(pkg, torch_gpu, send, counts, sdispls, elem, offsets=offsets,
+                                       tune=(0, -1, 0, -1, 0, defer))
     exp, erc, erd = alltoallv(oracle, send, counts, sdispls, elem, counts.sum(axis=0))
     assert nbytes == int(counts.sum()) * elem
     assert np.array_equal(rc, erc) and np.array_equal(rd, erd)
@@ -82,7 +84,9 @@ def test_a2av_random(P, elem, maxc, pkg, oracle, torch_gpu):
         assert np.array_equal(got[d], exp[d][:got[d].size]), f"rank {d} payload differs"
 
 
-@pytest.mark.parametrize("tune", [(256, 0, 2, 0, 1), (256, 16, 2, 1, 2), (256, 1, 8, 1, 1), (256, 4, 4, 0, 4), (256, 0, 4, 1, 8)])
+@pytest.mark.parametrize("tune", [(256, 0, 2, 0, 1, 0), (256, 16, 2, 1, 2, 0), (256, 1, 8, 1, 1, 0),
+                                  (256, 4, 4, 0, 4, 0), (256, 0, 4, 1, 8, 0), (256, 0, 2, 0, 1, 1),
+                                  (256, 16, 4, 1, 2, 1), (256, 1, 8, 1, 1, 1), (256, 0, 4, 1, 4, 1)])
 def test_a2av_launch_shapes(tune, pkg, oracle, torch_gpu):
     print(f"[gpu] a2av launch shape {tune}", flush=True)
     rng = np.random.default_rng(11)
@@ -105,10 +109,12 @@ def test_a2av_empty_and_single_heavy(pkg, oracle, torch_gpu):
     sdispls[:, 1:] = np.cumsum(counts, axis=1)[:, :-1]
     rng = np.random.default_rng(3)
     send = [rng.integers(0, 256, size=max(1, int(counts[s].sum())) * 4, dtype=np.uint8) for s in range(P)]
-    got, rc, rd, _ = run_exchange(pkg, torch_gpu, send, counts, sdispls, 4)
     exp, _, _ = alltoallv(oracle, send, counts, sdispls, 4, counts.sum(axis=0))
-    for d in range(P):
-        assert np.array_equal(got[d], exp[d][:got[d].size])
+    for defer in (0, 1):
+        got, rc, rd, _ = run_exchange(pkg, torch_gpu, send, counts, sdispls, 4,
+                                      tune=(0, -1, 0, -1, 0, defer))
+        for d in range(P):
+            assert np.array_equal(got[d], exp[d][:got[d].size]), f"defer={defer} rank {d}"
 
 
 def test_a2av_graph_and_autotune(pkg, oracle, torch_gpu):
