@@ -19,10 +19,13 @@
 //         the chunk's row and first segment (64-ary searches over L2-resident tables)
 //         and stages a window of 64 segments in LDS; each wave then copies a contiguous
 //         run of 64*U lines slice by slice, keeping a wave-uniform cursor into the window
-//         (it only moves forward) and choosing each lane's segment among the cursor's next
-//         four with compares.  A line inside one segment is one unaligned 16-byte load
-//         and one aligned 16-byte store; a line that straddles segments is four dword
-//         loads (issued with the other lines' loads) and one 16-byte store -- or, in
+//         (it only moves forward; kept in SGPRs) and, when the slice does not lie in one
+//         entry, choosing each lane's segment among the cursor's next four with compares.
+//         The loop is bound by instruction issue, so the common case -- a whole slice in
+//         one segment -- costs one uniform source offset.  A line inside one segment is
+//         one unaligned 16-byte load and one aligned 16-byte store; a line that straddles
+//         segments is four dword loads (issued with the other lines' loads) and one
+//         16-byte store -- or, in
 //         deferred mode (a tuning choice that pays when segments are short, as in skewed
 //         key distributions), left to one thread per segment run by the grid's first
 //         blocks, concurrently with the streaming blocks.  Lines at a row's ends, and
@@ -437,11 +440,13 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
             }
         }
         __syncthreads();
-        const int64_t d = s_meta[0];
-        const uint64_t rbase = (uint64_t)s_meta[1];
-        const int64_t rbytes = s_meta[2];
-        const uint64_t line1 = (uint64_t)s_meta[4];
-        uint64_t pos = (uint64_t)s_meta[3];
+        // block-uniform values read back from LDS: broadcast so that they live in SGPRs
+        // (the copy loop is bound by instruction issue; scalar arithmetic is free beside it)
+        const int64_t d = bcast(s_meta[0]);
+        const uint64_t rbase = (uint64_t)bcast(s_meta[1]);
+        const int64_t rbytes = bcast(s_meta[2]);
+        const uint64_t line1 = (uint64_t)bcast(s_meta[4]);
+        uint64_t pos = (uint64_t)bcast(s_meta[3]);
         while (pos < line1) {
             // ---- window: 64 segments from the one holding the row offset of line `pos`
             if (tid < kWave) {
@@ -489,11 +494,11 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
             w.st = s_st;
             w.en = s_en;
             w.D = s_D;
-            w.cnt = (int)s_meta[5];
-            w.nsearch = (int)s_meta[9];
-            const uint64_t pend = (uint64_t)s_meta[6];
-            w.cover = s_meta[7];
-            w.lo = s_meta[8];
+            w.cnt = (int)bcast(s_meta[5]);
+            w.nsearch = (int)bcast(s_meta[9]);
+            const uint64_t pend = (uint64_t)bcast(s_meta[6]);
+            w.cover = bcast(s_meta[7]);
+            w.lo = bcast(s_meta[8]);
             // each wave copies a contiguous run of 64*U lines, slice by slice, with a
             // wave-uniform cursor over the window (segments only move forward)
             const int wv = tid >> 6;
@@ -501,7 +506,7 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                 const uint64_t wb = base + (uint64_t)wv * kWave * U;
                 if (wb >= pend) continue;
                 int64_t off0 = (int64_t)(wb << 4) - (int64_t)rbase;
-                int kc = a2av_seg_lds(s_st, w.nsearch, off0 > w.lo ? off0 : w.lo);
+                int kc = bcast(a2av_seg_lds(s_st, w.nsearch, off0 > w.lo ? off0 : w.lo));
                 u32x4 v[U];
                 int kind[U];
 #pragma unroll
@@ -513,11 +518,21 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                     const int64_t soff = (int64_t)(sl << 4) - (int64_t)rbase;
                     // move the cursor to the entry holding the slice's first byte: a few
                     // steps forward, or a search when the slice starts further ahead
-                    if (kc + 3 < w.nsearch && soff >= s_en[kc + 3])
-                        kc = a2av_seg_lds(s_st, w.nsearch, soff);
+                    if (kc + 3 < w.nsearch && soff >= bcast(s_en[kc + 3]))
+                        kc = bcast(a2av_seg_lds(s_st, w.nsearch, soff));
                     else
-                        while (kc + 1 < w.nsearch && soff >= s_en[kc]) kc++;
+                        while (kc + 1 < w.nsearch && soff >= bcast(s_en[kc])) kc++;
                     const int64_t off = soff + 16 * l;
+                    const int64_t E0 = bcast(s_en[kc]);
+                    if (soff + 16 * kWave <= E0) {
+                        // the whole slice lies in entry kc (the common case): one uniform
+                        // source offset, no per-lane selection
+                        if (line < pend && off >= w.lo) {
+                            v[u] = gload16((line << 4) + bcast(s_D[kc]));
+                            kind[u] = 1;
+                        }
+                        continue;
+                    }
                     if (!(line < pend && off >= w.lo && off + 16 <= w.cover)) continue;
                     // this lane's entry: the cursor or one of the next three, else a search
                     int kb = kc;
