@@ -507,6 +507,8 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                 if (wb >= pend) continue;
                 int64_t off0 = (int64_t)(wb << 4) - (int64_t)rbase;
                 int kc = bcast(a2av_seg_lds(s_st, w.nsearch, off0 > w.lo ? off0 : w.lo));
+                int64_t Ekc = bcast(s_en[kc]);  // entry kc's end and source offset, in SGPRs
+                uint64_t Dkc = bcast(s_D[kc]);
                 u32x4 v[U];
                 int kind[U];
 #pragma unroll
@@ -518,17 +520,20 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                     const int64_t soff = (int64_t)(sl << 4) - (int64_t)rbase;
                     // move the cursor to the entry holding the slice's first byte: a few
                     // steps forward, or a search when the slice starts further ahead
-                    if (kc + 3 < w.nsearch && soff >= bcast(s_en[kc + 3]))
-                        kc = bcast(a2av_seg_lds(s_st, w.nsearch, soff));
-                    else
-                        while (kc + 1 < w.nsearch && soff >= bcast(s_en[kc])) kc++;
+                    if (soff >= Ekc && kc + 1 < w.nsearch) {
+                        if (kc + 3 < w.nsearch && soff >= bcast(s_en[kc + 3]))
+                            kc = bcast(a2av_seg_lds(s_st, w.nsearch, soff));
+                        else
+                            while (kc + 1 < w.nsearch && soff >= bcast(s_en[kc])) kc++;
+                        Ekc = bcast(s_en[kc]);
+                        Dkc = bcast(s_D[kc]);
+                    }
                     const int64_t off = soff + 16 * l;
-                    const int64_t E0 = bcast(s_en[kc]);
-                    if (soff + 16 * kWave <= E0) {
+                    if (soff + 16 * kWave <= Ekc) {
                         // the whole slice lies in entry kc (the common case): one uniform
-                        // source offset, no per-lane selection
+                        // source offset, no per-lane selection, no LDS access
                         if (line < pend && off >= w.lo) {
-                            v[u] = gload16((line << 4) + bcast(s_D[kc]));
+                            v[u] = gload16((line << 4) + Dkc);
                             kind[u] = 1;
                         }
                         continue;
