@@ -21,8 +21,9 @@
 //         run of 64*U lines slice by slice, keeping a wave-uniform cursor into the window
 //         (it only moves forward; kept in SGPRs) and, when the slice does not lie in one
 //         entry, choosing each lane's segment among the cursor's next four with compares.
-//         The loop is bound by instruction issue, so the common case -- a whole slice in
-//         one segment -- costs one uniform source offset.  A line inside one segment is
+//         The loop is bound by instruction issue, so the common cases are uniform paths:
+//         a slice inside one segment costs one source offset from SGPRs, a slice across
+//         one boundary a per-lane choice between two.  A line inside one segment is
 //         one unaligned 16-byte load and one aligned 16-byte store; a line that straddles
 //         segments is four dword loads (issued with the other lines' loads) and one
 //         16-byte store -- or, in
@@ -507,8 +508,10 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                 if (wb >= pend) continue;
                 int64_t off0 = (int64_t)(wb << 4) - (int64_t)rbase;
                 int kc = bcast(a2av_seg_lds(s_st, w.nsearch, off0 > w.lo ? off0 : w.lo));
-                int64_t Ekc = bcast(s_en[kc]);  // entry kc's end and source offset, in SGPRs
-                uint64_t Dkc = bcast(s_D[kc]);
+                // entries kc and kc + 1 (ends and source offsets), cached in SGPRs
+                int kc1 = kc + 1 < w.cnt ? kc + 1 : kc;
+                int64_t Ekc = bcast(s_en[kc]), Ekc1 = bcast(s_en[kc1]);
+                uint64_t Dkc = bcast(s_D[kc]), Dkc1 = bcast(s_D[kc1]);
                 u32x4 v[U];
                 int kind[U];
 #pragma unroll
@@ -525,8 +528,11 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                             kc = bcast(a2av_seg_lds(s_st, w.nsearch, soff));
                         else
                             while (kc + 1 < w.nsearch && soff >= bcast(s_en[kc])) kc++;
+                        kc1 = kc + 1 < w.cnt ? kc + 1 : kc;
                         Ekc = bcast(s_en[kc]);
                         Dkc = bcast(s_D[kc]);
+                        Ekc1 = bcast(s_en[kc1]);
+                        Dkc1 = bcast(s_D[kc1]);
                     }
                     const int64_t off = soff + 16 * l;
                     if (soff + 16 * kWave <= Ekc) {
@@ -535,6 +541,27 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
                         if (line < pend && off >= w.lo) {
                             v[u] = gload16((line << 4) + Dkc);
                             kind[u] = 1;
+                        }
+                        continue;
+                    }
+                    if (!DEFER && soff + 16 * kWave <= Ekc1) {
+                        // the slice spans entries kc and kc + 1: a lane's line lies in one of
+                        // them or straddles their boundary (one lane at most)
+                        if (line < pend && off >= w.lo) {
+                            const uint64_t la = line << 4;
+                            if (off + 16 <= Ekc || off >= Ekc) {
+                                v[u] = gload16(la + (off < Ekc ? Dkc : Dkc1));
+                                kind[u] = 1;
+                            } else if constexpr (!DEFER) {
+                                if (((Ekc - off) & 3) == 0 && ((la + Dkc) & 3) == 0 && ((la + Dkc1) & 3) == 0) {
+                                    v[u] = u32x4{
+                                        *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)(la + (off < Ekc ? Dkc : Dkc1)),
+                                        *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)(la + 4 + (off + 4 < Ekc ? Dkc : Dkc1)),
+                                        *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)(la + 8 + (off + 8 < Ekc ? Dkc : Dkc1)),
+                                        *(const GPUMPI_GLOBAL uint32_t*)(uintptr_t)(la + 12 + (off + 12 < Ekc ? Dkc : Dkc1))};
+                                    kind[u] = 1;
+                                }
+                            }
                         }
                         continue;
                     }
