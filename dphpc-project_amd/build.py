@@ -67,6 +67,37 @@ def build_programs(verbose=False):
     return outs
 
 
+# C MPI programs built through the converter and gpumpicc (rank arrays sized for 64 ranks):
+# (output name, sources relative to tests/programs/c, extra gpumpicc arguments)
+C_PROGRAMS = [
+    ("cpi", ["cpi.c"], []),
+    ("globals", ["globals.c"], []),
+    ("coalesce", ["coalesce.c"], ["--malloc-version", "5"]),
+    ("ring", ["proj/main.c", "proj/util.c"], []),
+    ("exits", ["exits.c"], []),
+]
+
+
+def build_c_programs(verbose=False):
+    src_dir = os.path.join(ROOT, "tests", "programs", "c")
+    out_dir = os.path.join(ROOT, "tests", "programs", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    env = dict(os.environ, GPU_MPI_MAX_RANKS="64")
+    outs = []
+    for name, srcs, extra in C_PROGRAMS:
+        out = os.path.join(out_dir, name)
+        cmd = [sys.executable, os.path.join(HERE, "gpumpicc.py"), *[os.path.join(src_dir, s) for s in srcs],
+               "-o", out, "--build-dir", os.path.join(out_dir, name + ".src"), *extra]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout)
+            raise RuntimeError(f"gpumpicc failed to build {name}")
+        outs.append(out)
+    return outs
+
+
 def build(verbose=False):
     """Compile libgpumpi.so (always recompiles); atomically replaces the old library."""
     fd, tmp = tempfile.mkstemp(prefix="libgpumpi.", suffix=".so.tmp", dir=HERE)
@@ -81,6 +112,7 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed to build libgpumpi.so")
     os.replace(tmp, LIB)
     build_programs(verbose)
+    build_c_programs(verbose)
     return LIB
 
 

@@ -85,6 +85,13 @@ __device__ int MPI_Abort(MPI_Comm, int errorcode) {
     record_error(kErrAbort, __LINE__, errorcode, "MPI_Abort called");
     return MPI_SUCCESS;
 }
+__device__ void gpumpi_exit(int status) {
+    Runtime* R = rt();
+    if (status != 0 && lane() == 0)
+        __hip_atomic_fetch_or(&R->exit_code, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    heap_flush_cache(&R->heap, &my_state()->cache);
+    __builtin_amdgcn_endpgm();
+}
 __device__ int MPI_Get_processor_name(char* name, int* resultlen) {
     const char n[] = "MI355X wavefront";
     for (int i = 0; i < (int)sizeof(n); i++) name[i] = n[i];

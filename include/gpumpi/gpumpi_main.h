@@ -9,6 +9,7 @@
 // runs __gpu_main on 64 wavefront ranks and exits with the program's status.
 #pragma once
 
+#include "libc/undef.h"
 #include "mpi.h"
 #include "global_vars.h"
 #include "gpumpi.h"

@@ -1,0 +1,73 @@
+// Device string.h for converted programs (gpu_libs/gpu_libc/string.cuh): the mem*
+// functions are the HIP device builtins; the str* functions are written here.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+namespace gpumpi_libc {
+__device__ inline size_t str_len(const char* s) {
+    size_t n = 0;
+    while (s[n]) n++;
+    return n;
+}
+__device__ inline char* str_cpy(char* d, const char* s) {
+    size_t i = 0;
+    for (; s[i]; i++) d[i] = s[i];
+    d[i] = 0;
+    return d;
+}
+__device__ inline char* str_ncpy(char* d, const char* s, size_t n) {
+    size_t i = 0;
+    for (; i < n && s[i]; i++) d[i] = s[i];
+    for (; i < n; i++) d[i] = 0;
+    return d;
+}
+__device__ inline int str_ncmp(const char* a, const char* b, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        const unsigned char x = (unsigned char)a[i], y = (unsigned char)b[i];
+        if (x != y) return x < y ? -1 : 1;
+        if (!x) return 0;
+    }
+    return 0;
+}
+__device__ inline int str_cmp(const char* a, const char* b) { return str_ncmp(a, b, (size_t)-1); }
+__device__ inline char* str_cat(char* d, const char* s) { str_cpy(d + str_len(d), s); return d; }
+__device__ inline char* str_ncat(char* d, const char* s, size_t n) {
+    char* e = d + str_len(d);
+    size_t i = 0;
+    for (; i < n && s[i]; i++) e[i] = s[i];
+    e[i] = 0;
+    return d;
+}
+__device__ inline char* str_chr(const char* s, int c) {
+    for (;; s++) {
+        if (*s == (char)c) return (char*)s;
+        if (!*s) return nullptr;
+    }
+}
+__device__ inline char* str_rchr(const char* s, int c) {
+    const char* r = nullptr;
+    for (;; s++) {
+        if (*s == (char)c) r = s;
+        if (!*s) return (char*)r;
+    }
+}
+__device__ inline char* str_str(const char* h, const char* n) {
+    const size_t m = str_len(n);
+    for (; *h; h++)
+        if (str_ncmp(h, n, m) == 0) return (char*)h;
+    return m == 0 ? (char*)h : nullptr;
+}
+}  // namespace gpumpi_libc
+
+#define GPUMPI_LIBC_STRING 1
+#define strlen(s) ::gpumpi_libc::str_len(s)
+#define strcpy(d, s) ::gpumpi_libc::str_cpy((d), (s))
+#define strncpy(d, s, n) ::gpumpi_libc::str_ncpy((d), (s), (n))
+#define strcmp(a, b) ::gpumpi_libc::str_cmp((a), (b))
+#define strncmp(a, b, n) ::gpumpi_libc::str_ncmp((a), (b), (n))
+#define strcat(d, s) ::gpumpi_libc::str_cat((d), (s))
+#define strncat(d, s, n) ::gpumpi_libc::str_ncat((d), (s), (n))
+#define strchr(s, c) ::gpumpi_libc::str_chr((s), (c))
+#define strrchr(s, c) ::gpumpi_libc::str_rchr((s), (c))
+#define strstr(h, n) ::gpumpi_libc::str_str((h), (n))

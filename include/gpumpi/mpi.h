@@ -123,6 +123,10 @@ __device__ int MPI_Init_thread(int* argc, char*** argv, int required, int* provi
 __device__ int MPI_Initialized(int* flag);
 __device__ int MPI_Finalize(void);
 __device__ int MPI_Abort(MPI_Comm comm, int errorcode);
+// exit(status) of a converted program (gpu_libs/gpu_libc/stdlib.cu:55-58): ends the
+// calling rank at once; a non-zero status makes the program's exit status 1, as a
+// non-zero return from main does.  Ranks still waiting on this one are not released.
+__device__ __attribute__((noreturn)) void gpumpi_exit(int status);
 __device__ int MPI_Get_processor_name(char* name, int* resultlen);
 __device__ double MPI_Wtime(void);
 __device__ double MPI_Wtick(void);

@@ -1,0 +1,60 @@
+"""C MPI programs converted by converter.py and built by gpumpicc.py (in build()), run
+through their own ---gpumpi command line like the reference's toolchain tests
+(toolchain_tests/pi/test.sh, global_var/test.sh).  Each program checks itself and exits
+with status 0 when every rank's checks hold."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+BUILD = os.path.join(ROOT, "tests", "programs", "_build")
+
+
+def run(name, args, timeout=120):
+    exe = os.path.join(BUILD, name)
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} was not built: run __graft_entry__.build()")
+    return subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.mark.parametrize("ranks", [1, 7, 64])
+def test_cpi(ranks):
+    print(f"[gpu] converted cpi on {ranks} ranks", flush=True)
+    r = run("cpi", ["20000", "---gpumpi", "-n", str(ranks), "-p", str(64 << 20), "-t", "60"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "pi is approximately 3.14159265" in r.stdout
+
+
+@pytest.mark.parametrize("ranks", [1, 16, 64])
+def test_globals_and_libc(ranks):
+    print(f"[gpu] converted globals on {ranks} ranks", flush=True)
+    r = run("globals", ["---gpumpi", "-n", str(ranks), "-p", str(64 << 20), "-t", "60"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"globals: ok on {ranks} ranks" in r.stdout
+    assert "rank check failed" not in r.stdout
+
+
+@pytest.mark.parametrize("ranks", [2, 8, 33])
+def test_coalesced_alltoall(ranks):
+    print(f"[gpu] converted coalesce on {ranks} ranks", flush=True)
+    r = run("coalesce", ["---gpumpi", "-n", str(ranks), "-p", str(64 << 20), "-t", "60"])
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("ranks", [2, 9])
+def test_two_file_ring(ranks):
+    print(f"[gpu] converted ring on {ranks} ranks", flush=True)
+    r = run("ring", ["---gpumpi", "-n", str(ranks), "-p", str(64 << 20), "-t", "60"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"ring: ok after {ranks} hops" in r.stdout
+
+
+def test_exit_ends_ranks_with_status():
+    print("[gpu] converted exit", flush=True)
+    r = run("exits", ["---gpumpi", "-n", "4", "-p", str(64 << 20), "-t", "60"])
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert "before exit" in r.stdout
+    assert "not reached" not in r.stdout
