@@ -235,8 +235,7 @@ class Converter:
             top = n["_parent"] == "TranslationUnitDecl"
             static_local = (not top) and n.get("storageClass") == "static"
             if top or static_local:
-                qt = n.get("type", {}).get("qualType", "")
-                if re.match(r"^(const\b|.*\bconst$)", qt) or qt.startswith("const "):
+                if self.is_const_object(n.get("type", {}).get("qualType", "")):
                     self.const_global_ids.add(n["id"])
                 else:
                     self.global_ids.add(n["id"])
@@ -245,6 +244,15 @@ class Converter:
             self.class_ids.add(n["id"])
         if k == "FunctionDecl" and self.project_node(n) and not n.get("isImplicit"):
             self.funcs.setdefault(n["name"], []).append(n)
+
+    @staticmethod
+    def is_const_object(qual_type):
+        """Whether the variable itself (or every array element) is const: `const int[4]`,
+        `char *const`, not `const char *[3]` (mutable pointers to const chars)."""
+        t = re.sub(r"(\s*\[\d*\])+$", "", qual_type).strip()
+        if "*" in t:
+            return t.endswith("const")
+        return t.startswith("const ") or " const" in t
 
     def name_end(self, n):
         l = n.get("_loc")
@@ -342,6 +350,12 @@ class Converter:
             return
         ne = self.name_end(n)
         self.edits.insert(ne, "[%d]" % self.N, rank=2, key="dim")
+        # `T name[] = {...}`: only the outer dimension of the per-rank array may be left
+        # open, so the size the initializer gave goes into the brackets
+        m = re.match(rb"\s*\[\s*\]", self.text(ne[0])[ne[1]:ne[1] + 64])
+        size = re.search(r"\[(\d+)\]", n.get("type", {}).get("qualType", ""))
+        if m and size:
+            self.edits.replace(ne[0], ne[1], ne[1] + m.end(), "[%s]" % size.group(1))
         if n.get("init") and n.get("inner"):
             if n["init"] != "c":
                 raise RuntimeError(f"only C-style initialization is supported for global {n.get('name')}")

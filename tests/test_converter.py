@@ -84,6 +84,18 @@ def test_globals_rewrite():
     assert "__gpu_global(kTable)" not in text
 
 
+def test_open_array_and_function_pointer_globals(tmp_path):
+    src = tmp_path / "g.c"
+    src.write_text("static int twice(int x) { return 2 * x; }\n"
+                   "const char *names[] = {\"a\", \"b\", \"c\"};\n"
+                   "int (*op)(int) = twice;\n"
+                   "int main(void) { return op(1) == 2 && names[2][0] == 'c' ? 0 : 1; }\n")
+    text = converter.convert_file(str(src), str(tmp_path), 0, (), 2)[os.path.realpath(str(src))].decode()
+    assert '__device__ const char *names[2][3] = {{"a", "b", "c"}, {"a", "b", "c"}};' in text
+    assert "__device__ int (*op[2])(int) = {twice, twice};" in text
+    assert "return __gpu_global(op)(1) == 2 && __gpu_global(names)[2][0] == 'c' ? 0 : 1;" in text
+
+
 def test_project_header_converted():
     _, out = convert("proj/util.c", ranks=4, project=os.path.join(C, "proj"))
     hdr = out[os.path.realpath(os.path.join(C, "proj", "util.h"))].decode()
