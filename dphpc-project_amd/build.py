@@ -75,6 +75,7 @@ C_PROGRAMS = [
     ("coalesce", ["coalesce.c"], ["--malloc-version", "5"]),
     ("ring", ["proj/main.c", "proj/util.c"], []),
     ("exits", ["exits.c"], []),
+    ("fileio", ["fileio.c"], []),
 ]
 
 

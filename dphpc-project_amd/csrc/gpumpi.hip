@@ -11,7 +11,12 @@
 #include "programs/serve.h"
 #include "../../include/gpumpi/gpumpi.h"
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -176,7 +181,10 @@ struct gpumpi_runtime {
     std::string err;
     bool needs_reset = false;
     int world_size = 0;  // ranks of MPI_COMM_WORLD as last set on the device
+    IoSlot* io = nullptr;  // pinned host I/O request slots, one per rank
+    class IoService* io_service = nullptr;
 };
+static void io_service_free(class IoService* s);
 
 static thread_local std::string g_host_error;
 
@@ -322,6 +330,20 @@ int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
     CommImpl* d_world;
     ALLOC(d_world, CommImpl, 1)
 #undef ALLOC
+    // host I/O request slots (device-to-host delegation of the C library's file calls)
+    if (hipHostMalloc((void**)&rt->io, sizeof(IoSlot) * (size_t)P,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        rt->io = nullptr;
+        set_err(rt, "hipHostMalloc of the I/O slots failed");
+        gpumpi_runtime_destroy(rt);
+        return GPUMPI_E_NOMEM;
+    }
+    std::memset(rt->io, 0, sizeof(IoSlot) * (size_t)P);
+    {
+        void* dio = nullptr;
+        if (hipHostGetDevicePointer(&dio, rt->io, 0) != hipSuccess) dio = rt->io;
+        h.io = (IoSlot*)dio;
+    }
     GroupImpl hg[2];
     hg[0].size = cfg.num_ranks;
     hg[0].refcount = -1;
@@ -368,6 +390,8 @@ void gpumpi_runtime_destroy(gpumpi_runtime* rt) {
     if (!rt) return;
     hipDeviceSynchronize();
     for (void* p : rt->allocs) hipFree(p);
+    io_service_free(rt->io_service);
+    if (rt->io) hipHostFree(rt->io);
     if (rt->d_args) hipFree(rt->d_args);
     if (rt->own_stream) hipStreamDestroy(rt->own_stream);
     delete rt;
@@ -466,6 +490,143 @@ int gpumpi_run_kernel(gpumpi_runtime* rt, const void* kernel, int argc, const ch
     return run_entry(rt, &pe, argc, argv, args, args_bytes, stream, elapsed_ms);
 }
 
+// ==================================================================== host I/O service
+// Serves the ranks' IoSlot requests while a rank grid runs (the device half is gpumpi_io
+// in runtime/mpi_api.h).  One thread per runtime, started with its first grid and parked
+// between grids, so a launch pays no thread start.  Files opened by ranks live in a
+// table; handles are 1-based indices, the standard streams have fixed negative handles.
+// Files still open when a grid ends are closed, as at process exit.
+class IoService {
+public:
+    IoService(IoSlot* slots, int n) : slots_(slots), n_(n) {
+        th_ = std::thread([this] { loop(); });
+    }
+    ~IoService() {
+        {
+            std::lock_guard<std::mutex> g(park_);
+            quit_ = true;
+        }
+        wake_.notify_all();
+        th_.join();
+    }
+    void begin() {
+        {
+            std::lock_guard<std::mutex> g(park_);
+            active_.store(true);
+        }
+        wake_.notify_all();
+    }
+    // After the grid has finished: serve what it posted last, close its files.
+    void end() {
+        active_.store(false);
+        std::lock_guard<std::mutex> g(serve_);
+        drain_locked();
+        for (FILE* f : files_)
+            if (f) std::fclose(f);
+        files_.clear();
+    }
+
+private:
+    FILE* file(int64_t h) {
+        if (h == kIoStdout) return stdout;
+        if (h == kIoStderr) return stderr;
+        if (h == kIoStdin) return stdin;
+        if (h < 1 || h > (int64_t)files_.size()) return nullptr;
+        return files_[(size_t)h - 1];
+    }
+    void serve(IoSlot& s) {
+        FILE* f = s.op == kIoOpen ? nullptr : file(s.a0);
+        int64_t r = -1;
+        switch (s.op) {
+        case kIoOpen: {
+            const char* path = s.data;
+            const size_t pl = strnlen(path, kIoData);
+            const char* mode = pl + 1 < (size_t)kIoData ? path + pl + 1 : "r";
+            FILE* nf = pl < (size_t)kIoData ? std::fopen(path, mode) : nullptr;
+            if (nf) {
+                files_.push_back(nf);
+                r = (int64_t)files_.size();
+            } else {
+                r = 0;
+            }
+            break;
+        }
+        case kIoClose:
+            if (f && s.a0 > 0) {
+                r = std::fclose(f);
+                files_[(size_t)s.a0 - 1] = nullptr;
+            }
+            break;
+        case kIoWrite:
+            if (f && s.len >= 0 && s.len <= kIoData) r = (int64_t)std::fwrite(s.data, 1, (size_t)s.len, f);
+            break;
+        case kIoRead:
+            if (f) {
+                const int64_t want = std::min<int64_t>(std::max<int64_t>(s.a1, 0), kIoData);
+                r = (int64_t)std::fread(s.data, 1, (size_t)want, f);
+            }
+            break;
+        case kIoGets:
+            if (f && s.a1 > 0) {
+                const int size = (int)std::min<int64_t>(s.a1, kIoData);
+                r = std::fgets(s.data, size, f) ? (int64_t)std::strlen(s.data) + 1 : -1;
+            }
+            break;
+        case kIoGetc: if (f) r = std::fgetc(f); break;
+        case kIoUngetc: if (f) r = std::ungetc((int)s.a1, f); break;
+        case kIoEof: if (f) r = std::feof(f); break;
+        case kIoFlush: if (f) r = std::fflush(f); break;
+        case kIoSeek: if (f) r = std::fseek(f, (long)s.a1, (int)s.a2); break;
+        case kIoTell: if (f) r = std::ftell(f); break;
+        default: break;
+        }
+        s.result = r;
+    }
+    bool drain() {
+        std::lock_guard<std::mutex> g(serve_);
+        return drain_locked();
+    }
+    bool drain_locked() {
+        bool any = false;
+        for (int i = 0; i < n_; i++) {
+            IoSlot& s = slots_[i];
+            const uint32_t seq = __atomic_load_n(&s.seq, __ATOMIC_ACQUIRE);
+            if (seq & 1) {
+                serve(s);
+                __atomic_store_n(&s.seq, seq + 1, __ATOMIC_RELEASE);
+                any = true;
+            }
+        }
+        return any;
+    }
+    void loop() {
+        int idle = 0;
+        for (;;) {
+            if (!active_.load()) {
+                std::unique_lock<std::mutex> lk(park_);
+                wake_.wait(lk, [this] { return quit_ || active_.load(); });
+                if (quit_) return;
+                idle = 0;
+            }
+            if (drain()) {
+                idle = 0;
+            } else if (++idle > 64) {
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        }
+    }
+    IoSlot* slots_;
+    int n_;
+    std::atomic<bool> active_{false};
+    bool quit_ = false;
+    std::mutex park_, serve_;
+    std::condition_variable wake_;
+    std::thread th_;
+    std::vector<FILE*> files_;
+};
+
+static void io_service_free(IoService* s) { delete s; }
+
 static int run_entry(gpumpi_runtime* rt, const ProgramEntry* pe, int argc,
                      const char* const* argv, const void* args, size_t args_bytes, void* stream,
                      double* elapsed_ms) {
@@ -500,9 +661,12 @@ static int run_entry(gpumpi_runtime* rt, const ProgramEntry* pe, int argc,
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s));
+    if (!rt->io_service) rt->io_service = new IoService(rt->io, rt->cfg.num_ranks);
+    rt->io_service->begin();
     rc = launch_program(rt, pe, argc, d_argv, s);
     HIP_TRY(hipEventRecord(e1, s));
     hipError_t se = hipStreamSynchronize(s);
+    rt->io_service->end();
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     hipEventDestroy(e0);

@@ -92,6 +92,52 @@ __device__ void gpumpi_exit(int status) {
     heap_flush_cache(&R->heap, &my_state()->cache);
     __builtin_amdgcn_endpgm();
 }
+// One request of this rank to the launcher's host I/O thread (see IoSlot).  Called
+// wave-uniformly; in1/in2 are copied back to back into the request, up to outcap bytes of
+// the answer into out (private, LDS-free memory handled like any message buffer).
+__device__ int64_t gpumpi_io(int op, int64_t a0, int64_t a1, int64_t a2, const void* in1,
+                             int len1, const void* in2, int len2, void* out, int outcap) {
+    Runtime* R = rt();
+    if (!R->io || len1 < 0 || len2 < 0 || len1 + len2 > kIoData) {
+        record_error(kErrArgument, __LINE__, len1 + len2, "host I/O request too large or no I/O channel");
+        return -1;
+    }
+    IoSlot* slot = R->io + world_rank();
+    uint32_t seq = 0;
+    if (lane() == 0) seq = __hip_atomic_load(&slot->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    seq = bcast(seq);
+    if (len1) wave_copy(slot->data, in1, (uint64_t)len1);
+    if (len2) wave_copy(slot->data + len1, in2, (uint64_t)len2);
+    if (lane() == 0) {
+        slot->op = op;
+        slot->a0 = a0;
+        slot->a1 = a1;
+        slot->a2 = a2;
+        slot->len = len1 + len2;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // the request before its seq, to the host
+    if (lane() == 0) __hip_atomic_store(&slot->seq, seq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    Waiter w;
+    bool answered = false;
+    for (;;) {
+        uint32_t s = 0;
+        if (lane() == 0) s = __hip_atomic_load(&slot->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (bcast(s) == seq + 2) {
+            answered = true;
+            break;
+        }
+        if (!w.keep_going(__LINE__)) break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+    if (!answered) return -1;
+    const int64_t res = bcast((uint64_t)slot->result);
+    if (out && outcap > 0 && (op == kIoRead || op == kIoGets) && res > 0) {
+        const int64_t n = res < outcap ? res : outcap;
+        wave_copy(out, slot->data, (uint64_t)n);
+    }
+    return res;
+}
+
 __device__ int MPI_Get_processor_name(char* name, int* resultlen) {
     const char n[] = "MI355X wavefront";
     for (int i = 0; i < (int)sizeof(n); i++) name[i] = n[i];

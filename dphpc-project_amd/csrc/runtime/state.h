@@ -15,6 +15,7 @@
 
 #include "common.h"
 #include "heap.h"
+#include "../../../include/gpumpi/io_ops.h"
 
 namespace gpumpi {
 
@@ -135,6 +136,21 @@ struct CommImpl {
     GroupImpl* group;     // group of this communicator (shared, read-only)
 };
 
+// ---------------------------------------------------------------- host I/O delegation
+// Each rank owns one request slot in pinned host memory: the rank writes the arguments
+// and up to kIoData bytes, then publishes an odd seq with a system-scope release; the
+// launcher's host thread performs the C library call and answers with seq + 1.  The
+// request codes (IoOp) are in include/gpumpi/io_ops.h.
+struct IoSlot {
+    uint32_t seq;
+    int32_t op;
+    int64_t a0, a1, a2;
+    int64_t result;
+    int32_t len;
+    int32_t pad;
+    char data[kIoData];
+};
+
 struct Runtime {
     int32_t nranks;
     int32_t ranks_per_block;
@@ -157,6 +173,7 @@ struct Runtime {
     const void* user_args;
     Heap heap;
     ErrorRecord err;
+    IoSlot* io;           // nranks request slots in pinned host memory
 };
 
 }  // namespace gpumpi

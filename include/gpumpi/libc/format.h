@@ -6,6 +6,7 @@
 #include <stdarg.h>
 #include <stddef.h>
 #include <stdint.h>
+#include "parse.h"
 
 #ifndef GPUMPI_LIBC_FN
 #define GPUMPI_LIBC_FN __device__ inline
@@ -275,6 +276,192 @@ GPUMPI_LIBC_FN int format(char* out, size_t cap, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     const int n = vformat(out, cap, fmt, ap);
+    va_end(ap);
+    return n;
+}
+
+// ---------------------------------------------------------------------------- scanning
+// The conversions of C99 scanf: d i u o x X f e g E G a s c n % and scansets [...],
+// '*' suppression, field widths and the lengths hh h l ll z j t L.  Src provides
+// get() (next byte, -1 at the end) and unget(byte).  Returns the number of assignments,
+// or -1 when the input ends before the first conversion.
+template <class Src>
+GPUMPI_LIBC_FN int vscan(Src& src, const char* fmt, va_list ap) {
+    int assigned = 0, count = 0;  // count: bytes consumed (%n)
+    bool any_conversion = false;
+    auto get = [&]() -> int { const int c = src.get(); if (c >= 0) count++; return c; };
+    auto unget = [&](int c) { if (c >= 0) { src.unget(c); count--; } };
+    auto space = [](int c) { return c == ' ' || (c >= '\t' && c <= '\r'); };
+    for (const char* p = fmt; *p; p++) {
+        if (space((unsigned char)*p)) {
+            int c;
+            while (space(c = get())) {}
+            unget(c);
+            continue;
+        }
+        if (*p != '%' || p[1] == '%') {
+            if (*p == '%') p++;
+            int c = get();
+            if (*p == '%') { while (space(c)) c = get(); }
+            if (c != (unsigned char)*p) {
+                unget(c);
+                if (c < 0 && !any_conversion) return -1;
+                return assigned;
+            }
+            continue;
+        }
+        p++;
+        const bool suppress = *p == '*';
+        if (suppress) p++;
+        int width = 0;
+        while (*p >= '0' && *p <= '9') width = width * 10 + (*p++ - '0');
+        int len = 0;  // -2 hh, -1 h, 0, 1 l, 2 ll, 3 L
+        for (;; p++) {
+            if (*p == 'h') len = len == -1 ? -2 : -1;
+            else if (*p == 'l') len = len == 1 ? 2 : 1;
+            else if (*p == 'L') len = 3;
+            else if (*p == 'z' || *p == 'j' || *p == 't') len = 2;
+            else break;
+        }
+        const char conv = *p;
+        if (!conv) break;
+        if (conv == 'n') {
+            if (!suppress) {
+                if (len == 2) *va_arg(ap, long long*) = count;
+                else if (len == 1) *va_arg(ap, long*) = count;
+                else *va_arg(ap, int*) = count;
+            }
+            continue;
+        }
+        int c;
+        if (conv != 'c' && conv != '[') {
+            while (space(c = get())) {}
+            unget(c);
+        }
+        if (width <= 0) width = conv == 'c' ? 1 : 1 << 30;
+        char buf[400];
+        int n = 0;
+        if (conv == 'c' || conv == 's' || conv == '[') {
+            bool set[256] = {};
+            bool neg = false;
+            if (conv == '[') {
+                p++;
+                if (*p == '^') { neg = true; p++; }
+                if (*p == ']') { set[(unsigned char)']'] = true; p++; }
+                for (; *p && *p != ']'; p++) {
+                    if (p[1] == '-' && p[2] && p[2] != ']') {
+                        for (int x = (unsigned char)p[0]; x <= (unsigned char)p[2]; x++) set[x] = true;
+                        p += 2;
+                    } else {
+                        set[(unsigned char)*p] = true;
+                    }
+                }
+                if (!*p) break;
+            }
+            char* dst = suppress ? nullptr : va_arg(ap, char*);
+            while (n < width) {
+                c = get();
+                if (c < 0) break;
+                const bool ok = conv == 'c' ? true : conv == 's' ? !space(c) : (set[c] != neg);
+                if (!ok) { unget(c); break; }
+                if (dst) dst[n] = (char)c;
+                n++;
+            }
+            if (n == 0 || (conv == 'c' && n < width)) {  // input or matching failure
+                if (c < 0 && !any_conversion) return -1;
+                return assigned;
+            }
+            if (dst && conv != 'c') dst[n] = 0;
+            any_conversion = true;
+            if (dst) assigned++;
+            continue;
+        }
+        // numbers: collect the longest valid prefix into buf
+        const bool flt = conv == 'f' || conv == 'e' || conv == 'g' || conv == 'E' ||
+                         conv == 'G' || conv == 'a' || conv == 'F';
+        int base = conv == 'o' ? 8 : (conv == 'x' || conv == 'X' || conv == 'p') ? 16 : conv == 'i' ? 0 : 10;
+        auto take = [&](int ch) { if (n < 399 && n < width) { buf[n++] = (char)ch; return true; } return false; };
+        c = get();
+        if ((c == '+' || c == '-') && take(c)) c = get();
+        if (!flt) {
+            if ((base == 0 || base == 16) && c == '0' && n < width) {
+                take(c);
+                c = get();
+                if ((c == 'x' || c == 'X') && n < width) {
+                    const int c2 = get();
+                    const bool hex = (c2 >= '0' && c2 <= '9') || (c2 >= 'a' && c2 <= 'f') || (c2 >= 'A' && c2 <= 'F');
+                    if (hex) { take(c); base = 16; c = c2; }
+                    else { unget(c2); }
+                } else if (base == 0) {
+                    base = 8;
+                }
+            }
+            if (base == 0) base = 10;
+            for (;; c = get()) {
+                int d = c >= '0' && c <= '9' ? c - '0' : c >= 'a' && c <= 'z' ? c - 'a' + 10 :
+                        c >= 'A' && c <= 'Z' ? c - 'A' + 10 : 99;
+                if (d >= base || n >= width || !take(c)) break;
+            }
+        } else {
+            bool digits = false;
+            while (c >= '0' && c <= '9' && take(c)) { digits = true; c = get(); }
+            if (c == '.' && take(c)) {
+                c = get();
+                while (c >= '0' && c <= '9' && take(c)) { digits = true; c = get(); }
+            }
+            if (digits && (c == 'e' || c == 'E') && n < width) {
+                const int save = n;
+                take(c);
+                int c2 = get();
+                if ((c2 == '+' || c2 == '-') && take(c2)) c2 = get();
+                bool ed = false;
+                while (c2 >= '0' && c2 <= '9' && take(c2)) { ed = true; c2 = get(); }
+                if (!ed) n = save;  // no exponent digits: not part of the number
+                c = c2;
+            }
+        }
+        unget(c);
+        buf[n] = 0;
+        const bool sign_only = n == 1 && (buf[0] == '+' || buf[0] == '-');
+        if (n == 0 || sign_only || (n == 2 && buf[1] == '.' && !(buf[0] >= '0' && buf[0] <= '9')) ||
+            (n == 1 && buf[0] == '.')) {
+            if (n == 0 && c < 0 && !any_conversion) return -1;
+            return assigned;
+        }
+        any_conversion = true;
+        if (suppress) continue;
+        if (flt) {
+            const double v = parse_double(buf, nullptr);
+            if (len >= 1) *va_arg(ap, double*) = v;
+            else *va_arg(ap, float*) = (float)v;
+        } else if (conv == 'p') {
+            *va_arg(ap, void**) = (void*)(uintptr_t)parse_int(buf, nullptr, 16);
+        } else {
+            const long long v = parse_int(buf, nullptr, base);
+            if (len == 2) *va_arg(ap, long long*) = v;
+            else if (len == 1) *va_arg(ap, long*) = (long)v;
+            else if (len == -1) *va_arg(ap, short*) = (short)v;
+            else if (len == -2) *va_arg(ap, signed char*) = (signed char)v;
+            else *va_arg(ap, int*) = (int)v;
+        }
+        assigned++;
+    }
+    return assigned;
+}
+
+// A string as a scanning source (sscanf).
+struct StringSource {
+    const char* s;
+    size_t i;
+    GPUMPI_LIBC_FN int get() { return s[i] ? (unsigned char)s[i++] : -1; }
+    GPUMPI_LIBC_FN void unget(int) { i--; }
+};
+
+GPUMPI_LIBC_FN int scan_string(const char* str, const char* fmt, ...) {
+    StringSource src{str, 0};
+    va_list ap;
+    va_start(ap, fmt);
+    const int n = vscan(src, fmt, ap);
     va_end(ap);
     return n;
 }

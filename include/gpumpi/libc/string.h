@@ -1,5 +1,6 @@
-// Device string.h for converted programs (gpu_libs/gpu_libc/string.cuh): the mem*
-// functions are the HIP device builtins; the str* functions are written here.
+// Device string.h for converted programs (gpu_libs/gpu_libc/string.cuh): memcpy and
+// memset are the HIP device builtins; memcmp, memchr, memmove and the str* functions are
+// written here.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -58,9 +59,35 @@ __device__ inline char* str_str(const char* h, const char* n) {
         if (str_ncmp(h, n, m) == 0) return (char*)h;
     return m == 0 ? (char*)h : nullptr;
 }
+__device__ inline int mem_cmp(const void* a, const void* b, size_t n) {
+    const unsigned char* x = (const unsigned char*)a;
+    const unsigned char* y = (const unsigned char*)b;
+    for (size_t i = 0; i < n; i++)
+        if (x[i] != y[i]) return x[i] < y[i] ? -1 : 1;
+    return 0;
+}
+__device__ inline void* mem_chr(const void* s, int c, size_t n) {
+    const unsigned char* p = (const unsigned char*)s;
+    for (size_t i = 0; i < n; i++)
+        if (p[i] == (unsigned char)c) return (void*)(p + i);
+    return nullptr;
+}
+__device__ inline void* mem_move(void* d, const void* s, size_t n) {
+    unsigned char* dp = (unsigned char*)d;
+    const unsigned char* sp = (const unsigned char*)s;
+    if (dp < sp) {
+        for (size_t i = 0; i < n; i++) dp[i] = sp[i];
+    } else if (dp > sp) {
+        for (size_t i = n; i-- > 0;) dp[i] = sp[i];
+    }
+    return d;
+}
 }  // namespace gpumpi_libc
 
 #define GPUMPI_LIBC_STRING 1
+#define memcmp(a, b, n) ::gpumpi_libc::mem_cmp((a), (b), (n))
+#define memchr(s, c, n) ::gpumpi_libc::mem_chr((s), (c), (n))
+#define memmove(d, s, n) ::gpumpi_libc::mem_move((d), (s), (n))
 #define strlen(s) ::gpumpi_libc::str_len(s)
 #define strcpy(d, s) ::gpumpi_libc::str_cpy((d), (s))
 #define strncpy(d, s, n) ::gpumpi_libc::str_ncpy((d), (s), (n))

@@ -127,6 +127,11 @@ __device__ int MPI_Abort(MPI_Comm comm, int errorcode);
 // calling rank at once; a non-zero status makes the program's exit status 1, as a
 // non-zero return from main does.  Ranks still waiting on this one are not released.
 __device__ __attribute__((noreturn)) void gpumpi_exit(int status);
+// File I/O of converted programs, delegated to the launcher's host thread: one request
+// (IoOp codes of the runtime) with up to 4096 bytes each way; returns the C library
+// call's result, -1 on failure.
+__device__ int64_t gpumpi_io(int op, int64_t a0, int64_t a1, int64_t a2, const void* in1,
+                             int len1, const void* in2, int len2, void* out, int outcap);
 __device__ int MPI_Get_processor_name(char* name, int* resultlen);
 __device__ double MPI_Wtime(void);
 __device__ double MPI_Wtick(void);

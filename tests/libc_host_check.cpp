@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <string>
 
 static int fails = 0;
 
@@ -67,6 +68,67 @@ int main() {
         const double a = gpumpi_libc::parse_double(s, &e1), b = std::strtod(s, &e2);
         if ((a != b && std::fabs(a - b) > 1e-15 * std::fabs(b)) || e1 != e2) {
             std::printf("parse_double %s: ours %.17g libc %.17g\n", s, a, b);
+            fails++;
+        }
+    }
+    // scanning: the same inputs through vscan and the C library's sscanf
+    struct ScanCase { const char* in; const char* fmt; };
+    const ScanCase scans[] = {
+        {"42 -17 0x1f 0755", "%d %d %i %i"}, {"  12abc", "%d%s"}, {"3.5e2 -0.25 7", "%lf %f %lf"},
+        {"a,b,c", "%c,%c,%c"}, {"key=value;rest", "%[^=]=%[^;];%s"}, {"12345", "%3d%2d"},
+        {"x 5", "%d"}, {"", "%d"}, {"   ", "%d"}, {"10 20", "%*d %d"}, {"abc 99", "abc %d"},
+        {"1.5x", "%lf%c"}, {"-", "%d"}, {"ff 17", "%x %o"}, {"9 %", "%d %%"}, {"1e", "%lf%s"},
+        {"+7 -0 .5 5.", "%d %d %lf %lf"}, {"abc", "%2s%s"}, {"Class = S", " Class = %c"},
+        {"  2.5  3 end", "%lf %d %n"},
+    };
+    for (const ScanCase& t : scans) {
+        int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+        double da[4] = {0, 0, 0, 0}, db[4] = {0, 0, 0, 0};
+        float fa = 0, fb = 0;
+        char sa[3][64] = {{0}}, sb[3][64] = {{0}};
+        char ca[3] = {0, 0, 0}, cb[3] = {0, 0, 0};
+        int r1, r2;
+        const std::string f = t.fmt;
+        if (f == "%lf %f %lf") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &da[0], &fa, &da[1]);
+            r2 = std::sscanf(t.in, t.fmt, &db[0], &fb, &db[1]);
+        } else if (f == "%c,%c,%c") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &ca[0], &ca[1], &ca[2]);
+            r2 = std::sscanf(t.in, t.fmt, &cb[0], &cb[1], &cb[2]);
+        } else if (f == "%[^=]=%[^;];%s" || f == "%2s%s") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, sa[0], sa[1], sa[2]);
+            r2 = std::sscanf(t.in, t.fmt, sb[0], sb[1], sb[2]);
+        } else if (f == "%d%s" || f == "%lf%s") {
+            if (f == "%d%s") {
+                r1 = gpumpi_libc::scan_string(t.in, t.fmt, &a[0], sa[0]);
+                r2 = std::sscanf(t.in, t.fmt, &b[0], sb[0]);
+            } else {
+                r1 = gpumpi_libc::scan_string(t.in, t.fmt, &da[0], sa[0]);
+                r2 = std::sscanf(t.in, t.fmt, &db[0], sb[0]);
+            }
+        } else if (f == "%lf%c") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &da[0], &ca[0]);
+            r2 = std::sscanf(t.in, t.fmt, &db[0], &cb[0]);
+        } else if (f == "%d %d %lf %lf") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &a[0], &a[1], &da[0], &da[1]);
+            r2 = std::sscanf(t.in, t.fmt, &b[0], &b[1], &db[0], &db[1]);
+        } else if (f == " Class = %c") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &ca[0]);
+            r2 = std::sscanf(t.in, t.fmt, &cb[0]);
+        } else if (f == "%lf %d %n") {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &da[0], &a[0], &a[1]);
+            r2 = std::sscanf(t.in, t.fmt, &db[0], &b[0], &b[1]);
+        } else {
+            r1 = gpumpi_libc::scan_string(t.in, t.fmt, &a[0], &a[1], &a[2], &a[3]);
+            r2 = std::sscanf(t.in, t.fmt, &b[0], &b[1], &b[2], &b[3]);
+        }
+        const bool same = r1 == r2 && std::memcmp(a, b, sizeof a) == 0 && std::memcmp(da, db, sizeof da) == 0 &&
+                          fa == fb && std::memcmp(sa, sb, sizeof sa) == 0 && std::memcmp(ca, cb, sizeof ca) == 0;
+        if (!same) {
+            std::printf("scan \"%s\" with \"%s\": ours %d [%d %d %d %d | %g %g %g | %s %s %s | %c%c%c], "
+                        "libc %d [%d %d %d %d | %g %g %g | %s %s %s | %c%c%c]\n", t.in, t.fmt,
+                        r1, a[0], a[1], a[2], a[3], da[0], da[1], (double)fa, sa[0], sa[1], sa[2], ca[0], ca[1], ca[2],
+                        r2, b[0], b[1], b[2], b[3], db[0], db[1], (double)fb, sb[0], sb[1], sb[2], cb[0], cb[1], cb[2]);
             fails++;
         }
     }
