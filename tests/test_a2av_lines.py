@@ -157,16 +157,26 @@ def copy_kernel(P, seg_end, seg_src, row_bytes, recv_addr, chunk_off, C, U, defe
                     if wb >= pend:
                         continue
                     kc = lds(max((wb << 4) - rbase, lo)) if not slow else 0
+                    # entries kc and kc + 1 cached per wave (bounds and source offsets)
+                    kc1 = kc + 1 if kc + 1 < cnt else kc
+                    Ekc, Dkc = ens[kc], Ds[kc]
+                    Ekc1, Dkc1 = ens[kc1], Ds[kc1]
                     for u in range(U):
                         sl = wb + u * 64
                         if sl >= pend:
                             continue
                         soff = (sl << 4) - rbase
-                        if kc + 3 < nsearch and soff >= ens[kc + 3]:
-                            kc = lds(soff)
-                        else:
-                            while kc + 1 < nsearch and soff >= ens[kc]:
-                                kc += 1
+                        if soff >= Ekc and kc + 1 < nsearch:
+                            if kc + 3 < nsearch and soff >= ens[kc + 3]:
+                                kc = lds(soff)
+                            else:
+                                while kc + 1 < nsearch and soff >= ens[kc]:
+                                    kc += 1
+                            kc1 = kc + 1 if kc + 1 < cnt else kc
+                            Ekc, Dkc = ens[kc], Ds[kc]
+                            Ekc1, Dkc1 = ens[kc1], Ds[kc1]
+                        one = soff + 16 * 64 <= Ekc
+                        two = not one and not defer and soff + 16 * 64 <= Ekc1
                         for l in range(64):
                             line = sl + l
                             if line >= pend:
@@ -174,6 +184,33 @@ def copy_kernel(P, seg_end, seg_src, row_bytes, recv_addr, chunk_off, C, U, defe
                             seen.append(line)
                             off = soff + 16 * l
                             la = line << 4
+                            if one:
+                                # the slice lies in entry kc: one source offset for every lane
+                                if off >= lo:
+                                    assert win[kc][0] <= off and off + 16 <= win[kc][1]
+                                    for b in range(16):
+                                        mem.write(la + b, mem.src(la + b + Dkc), d)
+                                else:
+                                    boundary(line)
+                                continue
+                            if two:
+                                # the slice spans entries kc and kc + 1
+                                if off < lo:
+                                    boundary(line)
+                                elif off + 16 <= Ekc or off >= Ekc:
+                                    Dl = Dkc if off < Ekc else Dkc1
+                                    k = kc if off < Ekc else kc1
+                                    assert win[k][0] <= off and off + 16 <= win[k][1]
+                                    for b in range(16):
+                                        mem.write(la + b, mem.src(la + b + Dl), d)
+                                elif (Ekc - off) % 4 == 0 and (la + Dkc) % 4 == 0 and (la + Dkc1) % 4 == 0:
+                                    for i in range(4):
+                                        Dq = Dkc if off + 4 * i < Ekc else Dkc1
+                                        for b in range(4):
+                                            mem.write(la + 4 * i + b, mem.src(la + 4 * i + b + Dq), d)
+                                else:
+                                    boundary(line)
+                                continue
                             if not (off >= lo and off + 16 <= cov):
                                 boundary(line)
                                 continue
