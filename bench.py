@@ -337,8 +337,9 @@ def main():
                     help="also run one 512 MiB device copy (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the exchange of one "
-                                   "GPU's ranks is measured (see DESIGN.md for sharding)",
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU: it measures the "
+                                   "exchange of the ranks one GPU holds (contiguous rank ranges "
+                                   "per GPU when sharded)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -21,8 +21,8 @@
  *   gpumpi_serve_*, gpumpi_run_job ... no reference counterpart: the device-resident job
  *                            server and the one-launch-per-job plain path
  *
- * A reference-side maintainer binds these with ctypes/cffi or links them directly; the
- * exact bindings are listed in INTEGRATION.md.
+ * A reference-side maintainer binds these with ctypes/cffi (dphpc-project_amd/__init__.py
+ * is such a binding) or links them directly (-I include -lgpumpi).
  */
 #ifndef GPUMPI_GPUMPI_H
 #define GPUMPI_GPUMPI_H
