@@ -14,8 +14,9 @@ roofline = the copy kernel alone (HIP events around it on its stream, averaged):
          same algorithmic bytes / its duration, against the 8 TB/s HBM peak.
 Also reported: the plain per-call path (one rank-grid kernel launch per MPI call:
 MPI_Alltoall then MPI_Alltoallv on 512 wavefront ranks) and the ratio, the Zipf(1.1)
-variant, the host-MPI CPU baseline (the same exchange under MPICH, one rank per host
-core, 2^18 keys per rank), and served MPI jobs: Poisson arrivals of independent dot /
+variant, the particle variant of the exchange (2^24 particles on an 8x8x8 rank grid, as a
+rank-grid MPI program), the host-MPI CPU baseline (the same exchange under MPICH, one rank
+per host core, 2^18 keys per rank), and served MPI jobs: Poisson arrivals of independent dot /
 Alltoall jobs of 8..256 ranks at fractions of the plain path's capacity, p50/p99
 latency (arrival -> result on the host) and jobs/s for the device-resident server
 versus one plain rank-grid launch per job; every served result is compared with the
@@ -200,6 +201,45 @@ def make_jobs(pkg, njobs, seed, heap_budget):
     return jobs
 
 
+def particle_exchange(pkg, torch, args):
+    """The particle variant of the exchange on the rank grid: 2^24 particles of 8 FP64
+    fields on an 8x8x8 periodic rank grid (32768 per rank), ~5 % moving to a face
+    neighbour per step, as Alltoall(counts) + Alltoallv (mode 0) and as neighbour
+    Isend/Irecv (mode 1).  The per-step time is the difference of an 11-step and a 1-step
+    launch (so launch and initialisation cancel); GB/s counts each migrating particle's
+    64 bytes read once and written once."""
+    from dphpc_project_amd import programs as pr
+    gx = gy = gz = 8
+    P, n0, cap = 512, 32768, 40960
+    out = {"workload": "2^24 particles x 8 FP64 fields, 8x8x8 periodic rank grid, Bernoulli(0.05) "
+                       "moves to one of 6 face neighbours per step (rank-grid MPI program)"}
+    bufs = [torch.zeros(P * cap * 8, dtype=torch.float64, device="cuda") for _ in range(2)]
+    send = torch.zeros(P * cap * 8, dtype=torch.float64, device="cuda")
+    counts = torch.zeros(2 * P, dtype=torch.int64, device="cuda")
+    cnt = torch.zeros(4 * P * P, dtype=torch.int32, device="cuda")
+    moved = torch.zeros(P, dtype=torch.int64, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=60.0) as rt:
+        for mode, name in ((0, "alltoallv"), (1, "neighbours")):
+            t, m = {}, {}
+            for steps in (1, 11):
+                moved.zero_()
+                a = pr.ParticleArgs(0x5EED, gx, gy, gz, mode, steps, 0, n0, cap,
+                                    (pr._P * 2)(bufs[0].data_ptr(), bufs[1].data_ptr()),
+                                    send.data_ptr(), counts.data_ptr(), cnt.data_ptr(),
+                                    moved.data_ptr(), sec.data_ptr())
+                _, t[steps] = rt.run("particles", a)
+                torch.cuda.synchronize()
+                m[steps] = int(moved.sum().item())
+            step_ms = (t[11] - t[1]) / 10
+            per_step = (m[11] - m[1]) / 10
+            out[name] = {"ms_per_step": round(step_ms, 4), "migrants_per_step": round(per_step),
+                         "GBps": round(2 * 64 * per_step / (step_ms * 1e-3) / 1e9, 2)}
+            log(f"particles ({name}): {step_ms:.3f} ms/step, {per_step:.0f} migrants/step, "
+                f"{out[name]['GBps']} GB/s")
+    return out
+
+
 def serve_latency(pkg, torch, args):
     """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
     the host, for one plain rank-grid launch per job and for the device-resident server.
@@ -328,6 +368,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 2.0])
@@ -435,6 +476,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
 
+    particles = None if args.no_particles else particle_exchange(pkg, torch, args)
     serve = None if args.no_serve else serve_latency(pkg, torch, args)
 
     cpu = None if args.no_cpu else cpu_baseline(args)
@@ -473,6 +515,7 @@ def main():
         "plain_path": plain,
         "batched_vs_plain": round(plain["ms_per_step"] / ms, 2) if plain else None,
         "zipf": zipf,
+        "particles": particles,
         "serving": serve,
     }
     print(json.dumps(result), flush=True)
