@@ -202,17 +202,23 @@ def make_jobs(pkg, njobs, seed, heap_budget):
 
 
 def particle_exchange(pkg, torch, args):
-    """The particle variant of the exchange on the rank grid: 2^24 particles of 8 FP64
-    fields on an 8x8x8 periodic rank grid (32768 per rank), ~5 % moving to a face
-    neighbour per step, as Alltoall(counts) + Alltoallv (mode 0) and as neighbour
-    Isend/Irecv (mode 1).  The per-step time is the difference of an 11-step and a 1-step
-    launch (so launch and initialisation cancel); GB/s counts each migrating particle's
-    64 bytes read once and written once."""
+    """The particle variant of the exchange: 2^24 particles of 8 FP64 fields on an 8x8x8
+    periodic rank grid (32768 per rank), ~5 % moving to a face neighbour per step.
+
+    Rank-grid MPI program: Alltoall(counts) + Alltoallv (mode 0) and neighbour Isend/Irecv
+    (mode 1); its per-step time is the difference of an 11-step and a 1-step launch (so
+    launch and initialisation cancel).  Batched step (gpumpi_particles_*): the same 11
+    steps from the same initial state must end in the rank-grid program's state bit for
+    bit; then 10-step hipGraphs are timed with HIP events, and the scatter kernel alone
+    (events around it) gives the roofline: 2 x 64 B per particle plus two direction bytes.
+    Migrant GB/s count each migrating particle's 64 bytes read once and written once."""
+    import numpy as np
     from dphpc_project_amd import programs as pr
     gx = gy = gz = 8
     P, n0, cap = 512, 32768, 40960
+    seed = 0x5EED
     out = {"workload": "2^24 particles x 8 FP64 fields, 8x8x8 periodic rank grid, Bernoulli(0.05) "
-                       "moves to one of 6 face neighbours per step (rank-grid MPI program)"}
+                       "moves to one of 6 face neighbours per step"}
     bufs = [torch.zeros(P * cap * 8, dtype=torch.float64, device="cuda") for _ in range(2)]
     send = torch.zeros(P * cap * 8, dtype=torch.float64, device="cuda")
     counts = torch.zeros(2 * P, dtype=torch.int64, device="cuda")
@@ -224,7 +230,7 @@ def particle_exchange(pkg, torch, args):
             t, m = {}, {}
             for steps in (1, 11):
                 moved.zero_()
-                a = pr.ParticleArgs(0x5EED, gx, gy, gz, mode, steps, 0, n0, cap,
+                a = pr.ParticleArgs(seed, gx, gy, gz, mode, steps, 0, n0, cap,
                                     (pr._P * 2)(bufs[0].data_ptr(), bufs[1].data_ptr()),
                                     send.data_ptr(), counts.data_ptr(), cnt.data_ptr(),
                                     moved.data_ptr(), sec.data_ptr())
@@ -237,6 +243,67 @@ def particle_exchange(pkg, torch, args):
                          "GBps": round(2 * 64 * per_step / (step_ms * 1e-3) / 1e9, 2)}
             log(f"particles ({name}): {step_ms:.3f} ms/step, {per_step:.0f} migrants/step, "
                 f"{out[name]['GBps']} GB/s")
+    del cnt
+    # both modes end in the same state; the last run (11 steps) left it in bufs[1]
+    ref = (send.copy_(bufs[1]), counts[P:].clone())
+
+    # batched step: 11 steps from the same initial state, compared with the rank grid's
+    stream = torch.cuda.current_stream().cuda_stream
+    plan = pkg.ParticlePlan((gx, gy, gz), cap, seed)
+    A, B = bufs
+    ca = torch.zeros(P, dtype=torch.int64, device="cuda")
+    cb = torch.zeros(P, dtype=torch.int64, device="cuda")
+    plan.init(A.data_ptr(), ca.data_ptr(), n0, stream)
+    plan.prepare(A.data_ptr(), ca.data_ptr(), 0, stream)
+    for i in range(11):
+        x, cx, y, cy = (A, ca, B, cb) if i % 2 == 0 else (B, cb, A, ca)
+        plan.step(x.data_ptr(), cx.data_ptr(), y.data_ptr(), cy.data_ptr(), stream)
+    torch.cuda.synchronize()
+    mask = torch.arange(cap, device="cuda")[None, :] < cb[:, None]
+    same = bool(torch.equal(cb, ref[1])) and bool(torch.equal(
+        B.view(P, cap, 8)[mask].view(torch.int64), ref[0].view(P, cap, 8)[mask].view(torch.int64)))
+    del mask, ref
+    send = None
+    assert same, "batched particle step differs from the rank-grid MPI program"
+    m0 = plan.status()[0]
+    g = 10
+    plan.graph_replay(B.data_ptr(), cb.data_ptr(), A.data_ptr(), ca.data_ptr(), g, stream)  # warm-up
+    reps = 5
+    ms = time_steps(torch, lambda: plan.graph_replay(B.data_ptr(), cb.data_ptr(), A.data_ptr(),
+                                                     ca.data_ptr(), g, stream), reps) / g
+    m1, overflow, nxt = plan.status()
+    assert overflow == 0, "a rank exceeded its particle capacity"
+    per_step = (m1 - m0) / ((reps + 1) * g)
+    # kernel durations of plain steps (events around each kernel), averaged
+    plan.phase_times(True)
+    kt = np.zeros(4)
+    nk = 6
+    for i in range(nk):
+        x, cx, y, cy = (B, cb, A, ca) if i % 2 == 0 else (A, ca, B, cb)
+        plan.step(x.data_ptr(), cx.data_ptr(), y.data_ptr(), cy.data_ptr(), stream)
+        kt += np.array(plan.phase_times())
+    plan.phase_times(False)
+    kt /= nk
+    nparts = P * n0
+    scatter_bytes = nparts * (2 * 64 + 2)
+    scatter_gbps = scatter_bytes / (kt[3] * 1e-6) / 1e9
+    plan.close()
+    rg = out["alltoallv"]["ms_per_step"]
+    out["batched"] = {
+        "ms_per_step": round(ms, 4), "steps_timed": reps * g, "migrants_per_step": round(per_step),
+        "GBps_migrants": round(2 * 64 * per_step / (ms * 1e-3) / 1e9, 2),
+        "GBps_state": round(2 * 64 * nparts / (ms * 1e-3) / 1e9, 1),
+        "kernel_us": {"count": round(kt[0], 1), "scan": round(kt[1], 1), "place": round(kt[2], 1),
+                      "scatter": round(kt[3], 1)},
+        "scatter_roofline": {"bound": "hbm", "achieved": round(scatter_gbps, 1), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(scatter_gbps / HBM_PEAK_GBS, 4),
+                             "bytes_per_launch": scatter_bytes},
+        "matches_rank_grid_11_steps": same,
+        "vs_rank_grid_alltoallv": round(rg / ms, 2),
+    }
+    log(f"particles (batched): {ms:.3f} ms/step ({rg / ms:.1f}x the rank-grid Alltoallv step), "
+        f"scatter {kt[3]:.1f} us = {scatter_gbps:.0f} GB/s ({100 * scatter_gbps / HBM_PEAK_GBS:.1f} % of peak), "
+        f"count/scan/place {kt[0]:.1f}/{kt[1]:.1f}/{kt[2]:.1f} us; same state as the rank grid: {same}")
     return out
 
 

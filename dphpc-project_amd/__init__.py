@@ -100,6 +100,16 @@ _SIGNATURES = {
     "gpumpi_a2av_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_get_tune": (ctypes.c_int, [_P, ctypes.POINTER(A2avTune), ctypes.POINTER(_I32)]),
     "gpumpi_a2av_payload_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "gpumpi_particles_plan_create": (ctypes.c_int, [_I32, _I32, _I32, ctypes.c_int64, ctypes.c_uint64,
+                                                    _I32, ctypes.POINTER(_P)]),
+    "gpumpi_particles_plan_destroy": (None, [_P]),
+    "gpumpi_particles_init": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
+    "gpumpi_particles_prepare": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P]),
+    "gpumpi_particles_step": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "gpumpi_particles_graph_replay": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _P]),
+    "gpumpi_particles_status": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_I32),
+                                               ctypes.POINTER(ctypes.c_uint32)]),
+    "gpumpi_particles_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_serve_start": (ctypes.c_int, [ctypes.POINTER(Config), _I32, _I32, ctypes.c_double,
                                           ctypes.POINTER(_P)]),
     "gpumpi_serve_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job)]),
@@ -402,6 +412,73 @@ class A2avPlan:
     def close(self):
         if self.h:
             self.lib.gpumpi_a2av_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ParticlePlan:
+    """Batched particle step on a periodic gx*gy*gz rank grid (gpumpi_particles_*): count,
+    scan, place and scatter kernels per step, replayed as one hipGraph.  States are device
+    buffers parts[P][cap][8] (float64) and counts[P] (int64)."""
+
+    def __init__(self, dims, cap, seed, nontemporal=True):
+        self.lib = load_library()
+        gx, gy, gz = dims
+        h = _P()
+        rc = self.lib.gpumpi_particles_plan_create(gx, gy, gz, cap, seed, 1 if nontemporal else 0,
+                                                   ctypes.byref(h))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_particles_plan_create failed ({rc})")
+        self.h = h
+
+    @staticmethod
+    def _s(stream):
+        return ctypes.c_void_p(stream) if stream else None
+
+    def _check(self, rc, what):
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_particles_{what} failed ({rc})")
+
+    def init(self, parts_ptr, counts_ptr, n0, stream=0):
+        self._check(self.lib.gpumpi_particles_init(self.h, parts_ptr, counts_ptr, n0, self._s(stream)), "init")
+
+    def prepare(self, parts_ptr, counts_ptr, step, stream=0):
+        self._check(self.lib.gpumpi_particles_prepare(self.h, parts_ptr, counts_ptr, step, self._s(stream)),
+                    "prepare")
+
+    def step(self, in_ptr, cin_ptr, out_ptr, cout_ptr, stream=0):
+        self._check(self.lib.gpumpi_particles_step(self.h, in_ptr, cin_ptr, out_ptr, cout_ptr,
+                                                   self._s(stream)), "step")
+
+    def graph_replay(self, a_ptr, ca_ptr, b_ptr, cb_ptr, steps, stream=0):
+        self._check(self.lib.gpumpi_particles_graph_replay(self.h, a_ptr, ca_ptr, b_ptr, cb_ptr, steps,
+                                                           self._s(stream)), "graph_replay")
+
+    def status(self):
+        """(migrants since prepare, sticky overflow flag, index of the next step)."""
+        m, o, s = ctypes.c_int64(), _I32(), ctypes.c_uint32()
+        self._check(self.lib.gpumpi_particles_status(self.h, ctypes.byref(m), ctypes.byref(o),
+                                                     ctypes.byref(s)), "status")
+        return m.value, o.value, s.value
+
+    def phase_times(self, enable=None):
+        """enable=True/False arms/disarms event timing of step(); with no argument returns
+        [count_us, scan_us, place_us, scatter_us] of the last timed step."""
+        if enable is not None:
+            self.lib.gpumpi_particles_phase_times(self.h, 1 if enable else 0, None)
+            return None
+        out = (ctypes.c_double * 4)()
+        self._check(self.lib.gpumpi_particles_phase_times(self.h, -1, out), "phase_times")
+        return list(out)
+
+    def close(self):
+        if self.h:
+            self.lib.gpumpi_particles_plan_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -5,6 +5,7 @@
 #include "runtime/rank_kernel.h"
 #include "runtime/batched.h"
 #include "runtime/bucket.h"
+#include "runtime/particle_step.h"
 #include "programs/reference_tests.h"
 #include "programs/workloads.h"
 #include "programs/bucket_sort.h"
@@ -1086,6 +1087,230 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
     if (chosen) *chosen = pl->tune;
     if (best_us) *best_us = best_t;
     return rc;
+}
+
+}  // extern "C"
+
+// ==================================================================== batched particle step
+struct gpumpi_particle_plan {
+    PartStepDev dev{};
+    uint8_t* dirs[2] = {nullptr, nullptr};  // direction bytes of the current / next state
+    int cur = 0;
+    int nontemporal = 1;
+    std::vector<void*> allocs;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    int graph_steps = 0;
+    const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int graph_cur = -1;       // direction buffer the captured steps start from
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+static int ps_launch(gpumpi_particle_plan* pl, const double* in, const int64_t* cin, double* out,
+                     int64_t* cout, hipStream_t s, bool timed) {
+    PartStepDev& d = pl->dev;
+    d.in = in;
+    d.cin = cin;
+    d.dir_in = pl->dirs[pl->cur];
+    d.out = out;
+    d.cout = cout;
+    d.dir_out = pl->dirs[pl->cur ^ 1];
+    const int64_t waves = (int64_t)d.P * d.nchunks;
+    const dim3 chunk_grid((unsigned)((waves + 3) / 4)), rank_grid((unsigned)((d.P + 3) / 4)),
+        place_grid((unsigned)((d.P + kPsThreads - 1) / kPsThreads)), block(kPsThreads);
+    void* kargs[] = {(void*)&d};
+    const void* scatter = pl->nontemporal ? (const void*)&ps_scatter_kernel<true>
+                                          : (const void*)&ps_scatter_kernel<false>;
+    const void* kernels[4] = {(const void*)&ps_count_kernel, (const void*)&ps_scan_kernel,
+                              (const void*)&ps_place_kernel, scatter};
+    const dim3 grids[4] = {chunk_grid, rank_grid, place_grid, chunk_grid};
+    for (int i = 0; i < 4; i++) {
+        if (timed) hipEventRecord(pl->ev[i], s);
+        if (hipLaunchKernel(kernels[i], grids[i], block, kargs, 0, s) != hipSuccess)
+            return GPUMPI_E_HIP;
+    }
+    if (timed) hipEventRecord(pl->ev[4], s);
+    pl->cur ^= 1;
+    return GPUMPI_OK;
+}
+
+static void ps_drop_graph(gpumpi_particle_plan* pl) {
+    if (pl->graph) hipGraphExecDestroy(pl->graph);
+    if (pl->graph_def) hipGraphDestroy(pl->graph_def);
+    pl->graph = nullptr;
+    pl->graph_def = nullptr;
+    pl->graph_steps = 0;
+}
+
+extern "C" {
+
+int gpumpi_particles_plan_create(int32_t gx, int32_t gy, int32_t gz, int64_t cap, uint64_t seed,
+                                 int32_t nontemporal, gpumpi_particle_plan** out) {
+    if (!out || gx < 1 || gy < 1 || gz < 1 || cap < 1 || cap > 0x7FFFFFFF) return GPUMPI_E_ARG;
+    const int64_t P64 = (int64_t)gx * gy * gz;
+    if (P64 > (1 << 20)) return GPUMPI_E_ARG;
+    const int P = (int)P64;
+    const int64_t nchunks = (cap + kPsChunk - 1) / kPsChunk;
+    if ((P64 * nchunks + 3) / 4 > 0x7FFFFFFF || (double)P64 * (double)cap > 1e12) return GPUMPI_E_ARG;
+    gpumpi_particle_plan* pl = new gpumpi_particle_plan();
+    pl->nontemporal = nontemporal != 0;
+    PartStepDev& d = pl->dev;
+    d.gx = gx;
+    d.gy = gy;
+    d.gz = gz;
+    d.P = P;
+    d.cap = cap;
+    d.seed = seed;
+    d.nchunks = (int32_t)nchunks;
+    // host tables: neighbours and direction classes of every rank
+    std::vector<int32_t> nbr((size_t)P * kPsClasses, 0);
+    std::vector<uint32_t> cls((size_t)P);
+    for (int r = 0; r < P; r++) {
+        for (int dd = 0; dd < 6; dd++) nbr[(size_t)r * kPsClasses + dd] = grid_neighbour(r, dd, gx, gy, gz);
+        nbr[(size_t)r * kPsClasses + 6] = r;
+        nbr[(size_t)r * kPsClasses + 7] = r;
+        cls[r] = ps_classes(r, gx, gy, gz);
+    }
+    const size_t b_dirs = (size_t)P * (size_t)cap, b_chunk = sizeof(int32_t) * (size_t)P * nchunks * kPsClasses,
+                 b_rank = sizeof(int64_t) * (size_t)P * kPsClasses, b_nbr = nbr.size() * sizeof(int32_t),
+                 b_cls = cls.size() * sizeof(uint32_t);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t ws = 2 * al(b_dirs) + al(b_chunk) + 2 * al(b_rank) + al(b_nbr) + al(b_cls) + 256;
+    char* w = nullptr;
+    if (hipMalloc((void**)&w, ws) != hipSuccess) {
+        delete pl;
+        return GPUMPI_E_NOMEM;
+    }
+    pl->allocs.push_back(w);
+    hipMemset(w, 0, ws);
+    char* q = w;
+    pl->dirs[0] = (uint8_t*)q; q += al(b_dirs);
+    pl->dirs[1] = (uint8_t*)q; q += al(b_dirs);
+    d.chunk_off = (int32_t*)q; q += al(b_chunk);
+    d.tot = (int64_t*)q; q += al(b_rank);
+    d.base = (int64_t*)q; q += al(b_rank);
+    d.nbr_tab = (const int32_t*)q;
+    hipMemcpy(q, nbr.data(), b_nbr, hipMemcpyHostToDevice); q += al(b_nbr);
+    d.cls_tab = (const uint32_t*)q;
+    hipMemcpy(q, cls.data(), b_cls, hipMemcpyHostToDevice); q += al(b_cls);
+    d.stats = (int64_t*)q;
+    d.step = (uint32_t*)(q + 4 * sizeof(int64_t));
+    for (int i = 0; i < 5; i++) hipEventCreate(&pl->ev[i]);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        gpumpi_particles_plan_destroy(pl);
+        return GPUMPI_E_HIP;
+    }
+    *out = pl;
+    return GPUMPI_OK;
+}
+
+void gpumpi_particles_plan_destroy(gpumpi_particle_plan* pl) {
+    if (!pl) return;
+    hipDeviceSynchronize();
+    ps_drop_graph(pl);
+    for (hipEvent_t e : pl->ev)
+        if (e) hipEventDestroy(e);
+    for (void* p : pl->allocs) hipFree(p);
+    delete pl;
+}
+
+int gpumpi_particles_init(gpumpi_particle_plan* pl, double* parts, int64_t* counts, int64_t n0,
+                          void* stream) {
+    if (!pl || !parts || !counts || n0 < 0 || n0 > pl->dev.cap) return GPUMPI_E_ARG;
+    const PartStepDev& d = pl->dev;
+    const int64_t threads = std::max<int64_t>((int64_t)d.P * n0, d.P);
+    ps_init_kernel<<<dim3((unsigned)((threads + kPsThreads - 1) / kPsThreads)), dim3(kPsThreads), 0,
+                     (hipStream_t)stream>>>(d.gx, d.gy, d.gz, d.cap, n0, d.seed, parts, counts);
+    return hipGetLastError() == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_particles_prepare(gpumpi_particle_plan* pl, const double* parts, const int64_t* counts,
+                             uint32_t step, void* stream) {
+    if (!pl || !parts || !counts) return GPUMPI_E_ARG;
+    PartStepDev d = pl->dev;
+    d.in = parts;
+    d.cin = counts;
+    d.dir_out = pl->dirs[pl->cur];
+    const int64_t slots = (int64_t)d.P * d.cap;
+    ps_dirs_kernel<<<dim3((unsigned)((slots + kPsThreads - 1) / kPsThreads)), dim3(kPsThreads), 0,
+                     (hipStream_t)stream>>>(d, step);
+    return hipGetLastError() == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_particles_step(gpumpi_particle_plan* pl, const double* in, const int64_t* cin,
+                          double* out, int64_t* cout, void* stream) {
+    if (!pl || !in || !cin || !out || !cout || (const void*)in == (const void*)out) return GPUMPI_E_ARG;
+    return ps_launch(pl, in, cin, out, cout, (hipStream_t)stream, pl->timing);
+}
+
+int gpumpi_particles_graph_replay(gpumpi_particle_plan* pl, double* a, int64_t* ca, double* b,
+                                  int64_t* cb, int32_t steps, void* stream) {
+    if (!pl || !a || !ca || !b || !cb || a == b || steps <= 0 || (steps & 1)) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const void* key[4] = {a, ca, b, cb};
+    if (!pl->graph || pl->graph_steps != steps || pl->graph_cur != pl->cur ||
+        std::memcmp(key, pl->graph_key, sizeof(key)) != 0) {
+        ps_drop_graph(pl);
+        const int cur0 = pl->cur;
+        hipStream_t cs;
+        if (hipStreamCreate(&cs) != hipSuccess) return GPUMPI_E_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            hipStreamDestroy(cs);
+            return GPUMPI_E_HIP;
+        }
+        int rc = GPUMPI_OK;
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++)
+            rc = (i & 1) ? ps_launch(pl, b, cb, a, ca, cs, false) : ps_launch(pl, a, ca, b, cb, cs, false);
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(cs, &g);
+        hipStreamDestroy(cs);
+        if (rc != GPUMPI_OK || ee != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return GPUMPI_E_HIP;
+        }
+        pl->graph_def = g;
+        if (hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+            ps_drop_graph(pl);
+            return GPUMPI_E_HIP;
+        }
+        pl->graph_steps = steps;
+        pl->graph_cur = cur0;
+        std::memcpy(pl->graph_key, key, sizeof(key));
+    }
+    return hipGraphLaunch(pl->graph, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_particles_status(gpumpi_particle_plan* pl, int64_t* moved, int32_t* overflow,
+                            uint32_t* next_step) {
+    if (!pl) return GPUMPI_E_ARG;
+    if (hipDeviceSynchronize() != hipSuccess) return GPUMPI_E_HIP;
+    int64_t st[4];
+    uint32_t step = 0;
+    if (hipMemcpy(st, pl->dev.stats, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&step, pl->dev.step, sizeof(step), hipMemcpyDeviceToHost) != hipSuccess)
+        return GPUMPI_E_HIP;
+    if (moved) *moved = st[0];
+    if (overflow) *overflow = (int32_t)st[2];
+    if (next_step) *next_step = step;
+    return GPUMPI_OK;
+}
+
+int gpumpi_particles_phase_times(gpumpi_particle_plan* pl, int32_t enable, double* us) {
+    if (!pl) return GPUMPI_E_ARG;
+    if (enable >= 0) {
+        pl->timing = enable > 0;
+        return GPUMPI_OK;
+    }
+    if (!pl->timing || !us) return GPUMPI_E_ARG;
+    if (hipEventSynchronize(pl->ev[4]) != hipSuccess) return GPUMPI_E_HIP;
+    for (int i = 0; i < 4; i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pl->ev[i], pl->ev[i + 1]) != hipSuccess) return GPUMPI_E_HIP;
+        us[i] = 1000.0 * ms;
+    }
+    return GPUMPI_OK;
 }
 
 }  // extern "C"

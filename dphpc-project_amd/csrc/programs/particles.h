@@ -16,13 +16,10 @@
 #pragma once
 
 #include "workloads.h"
+#include "../runtime/particle_step.h"
 
 namespace gpumpi {
 namespace progs {
-
-constexpr uint32_t kStreamParticles = 0x0B17E507u;
-constexpr uint32_t kPartMove = 214748365u;
-constexpr int kPartFields = 8;
 
 struct ParticleArgs {
     uint64_t seed;
@@ -39,27 +36,6 @@ struct ParticleArgs {
     int64_t* moved;      // [P] migrants sent by the rank in this launch
     double* seconds;     // [P]
 };
-
-__host__ __device__ __forceinline__ int grid_neighbour(int r, int d, int gx, int gy, int gz) {
-    int x = r % gx, y = (r / gx) % gy, z = r / (gx * gy);
-    switch (d) {
-    case 0: x = x + 1 == gx ? 0 : x + 1; break;
-    case 1: x = x == 0 ? gx - 1 : x - 1; break;
-    case 2: y = y + 1 == gy ? 0 : y + 1; break;
-    case 3: y = y == 0 ? gy - 1 : y - 1; break;
-    case 4: z = z + 1 == gz ? 0 : z + 1; break;
-    default: z = z == 0 ? gz - 1 : z - 1; break;
-    }
-    return x + gx * (y + gy * z);
-}
-
-// Direction of particle `id` at `step`: -1 stays, 0..5 = +x -x +y -y +z -z.
-__host__ __device__ __forceinline__ int particle_dir(uint64_t seed, uint32_t step, uint64_t id) {
-    u32x4v o = philox4x32_10(u32x4v{(uint32_t)id, (uint32_t)(id >> 32), step, kStreamMigrate},
-                             (uint32_t)seed, (uint32_t)(seed >> 32));
-    if (o.x >= kPartMove) return -1;
-    return (int)(((uint64_t)o.y * 6u) >> 32);
-}
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -1,0 +1,369 @@
+// Batched particle step: the particle variant of the uneven exchange as one hipGraph-able
+// step of four kernels, with no MPI calls inside.
+//
+// Workload: P = gx*gy*gz ranks on a periodic grid; rank r owns the unit cell
+// (r % gx, (r / gx) % gy, r / (gx * gy)) and a slab of `cap` particle slots; a particle is
+// 8 FP64 fields {x, y, z, vx, vy, vz, w, id} (64 bytes).  Each step a particle moves one
+// cell along one of the six axis directions with probability kPartMove / 2^32, drawn from
+// Philox keyed by (seed, id, step).  The step's result is the one the MPI program gives
+// (Alltoall of the counts + MPI_Alltoallv, gpu_mpi/mpi.cu:333-408 semantics): in each
+// rank's new slab its stayers first, in order, then the arrivals in ascending source-rank
+// order, each source's migrants in their original order, the moved coordinate shifted by
+// +-1 and wrapped.
+//
+// One step, reading (in, cin, dir_in) and writing (out, cout, dir_out):
+//   ps_count_kernel    one wavefront per chunk of kPsChunk slots: counts per class (the
+//                      six directions merged where two reach the same rank, and stay)
+//   ps_scan_kernel     one wavefront per rank: exclusive offsets of every chunk within the
+//                      rank, per class, and the rank's totals
+//   ps_place_kernel    one thread per destination rank: the first slot of every arriving
+//                      (source, class) group, the new counts, the overflow flag, the step
+//   ps_scatter_kernel  one wavefront per chunk: moves every particle to its final slot,
+//                      shifts the migrants' coordinate, and writes the direction byte of the
+//                      particle's next step beside its new slot
+// dir_in[r*cap + k] is particle k's direction at this step (0..5, or kPsStay).  It is
+// written by the previous step's scatter (or by ps_dirs_kernel before the first step), so
+// the counting pass reads one byte per particle instead of the particle.
+//
+// HBM traffic per step: the particles read once and written once (2 x 64 B each) plus
+// 2 B of direction bytes; everything else is O(P * chunks).
+#pragma once
+
+#include "philox.h"
+#include "common.h"
+#include "copy.h"
+
+namespace gpumpi {
+
+constexpr uint32_t kStreamParticles = 0x0B17E507u;
+constexpr uint32_t kPartMove = 214748365u;   // ~0.05 * 2^32
+constexpr int kPartFields = 8;
+
+constexpr int kPsChunk = 256;     // particle slots per wavefront chunk
+constexpr int kPsThreads = 256;   // four chunks per workgroup
+constexpr int kPsStay = 6;        // direction byte of a particle that stays
+constexpr int kPsSkip = 7;        // internal: slot beyond the count, or a dropped migrant
+constexpr int kPsClasses = 8;     // per chunk / rank: six directions, stay, padding
+
+__host__ __device__ __forceinline__ int grid_neighbour(int r, int d, int gx, int gy, int gz) {
+    int x = r % gx, y = (r / gx) % gy, z = r / (gx * gy);
+    switch (d) {
+    case 0: x = x + 1 == gx ? 0 : x + 1; break;
+    case 1: x = x == 0 ? gx - 1 : x - 1; break;
+    case 2: y = y + 1 == gy ? 0 : y + 1; break;
+    case 3: y = y == 0 ? gy - 1 : y - 1; break;
+    case 4: z = z + 1 == gz ? 0 : z + 1; break;
+    default: z = z == 0 ? gz - 1 : z - 1; break;
+    }
+    return x + gx * (y + gy * z);
+}
+
+// Direction of particle `id` at `step`: -1 stays, 0..5 = +x -x +y -y +z -z.
+__host__ __device__ __forceinline__ int particle_dir(uint64_t seed, uint32_t step, uint64_t id) {
+    u32x4v o = philox4x32_10(u32x4v{(uint32_t)id, (uint32_t)(id >> 32), step, kStreamMigrate},
+                             (uint32_t)seed, (uint32_t)(seed >> 32));
+    if (o.x >= kPartMove) return -1;
+    return (int)(((uint64_t)o.y * 6u) >> 32);
+}
+
+// Class of each direction of rank r: the smallest direction reaching the same rank (on a
+// grid axis of length 1 or 2 several directions reach one rank, possibly r itself), four
+// bits per direction.
+__host__ __device__ __forceinline__ uint32_t ps_classes(int r, int gx, int gy, int gz) {
+    uint32_t packed = 0;
+    for (int d = 0; d < 6; d++) {
+        const int nb = grid_neighbour(r, d, gx, gy, gz);
+        int c = d;
+        for (int e = d - 1; e >= 0; e--)
+            if (grid_neighbour(r, e, gx, gy, gz) == nb) c = e;
+        packed |= (uint32_t)c << (4 * d);
+    }
+    return packed | ((uint32_t)kPsStay << 24);   // direction byte kPsStay -> class 6
+}
+
+struct PartStepDev {
+    int32_t gx, gy, gz, P;
+    int64_t cap;             // slots per rank slab
+    uint64_t seed;
+    int32_t nchunks;         // chunks per rank: ceil(cap / kPsChunk)
+    int32_t pad;
+    const double* in;        // [P][cap][8]
+    const int64_t* cin;      // [P]
+    const uint8_t* dir_in;   // [P][cap]
+    double* out;
+    int64_t* cout;
+    uint8_t* dir_out;
+    const uint32_t* cls_tab; // [P]: ps_classes of every rank
+    const int32_t* nbr_tab;  // [P][8]: grid_neighbour of every rank in directions 0..5
+    int32_t* chunk_off;      // [P][nchunks][8]: counts, then exclusive offsets in the rank
+    int64_t* tot;            // [P][8]: per-rank totals per class
+    int64_t* base;           // [P][8]: first slot of (source, class) in its destination slab
+    uint32_t* step;          // [0]: the step dir_in belongs to (the place kernel advances it)
+    int64_t* stats;          // [0] migrants since prepare, [1] overflow in this step, [2] sticky
+};
+
+__device__ __forceinline__ int64_t ps_count_of(const PartStepDev& a, int r) {
+    const int64_t n = a.cin[r];
+    return n < 0 ? 0 : n > a.cap ? a.cap : n;
+}
+
+__device__ __forceinline__ int ps_class_of(uint32_t packed, int d) {
+    d = d > kPsStay ? kPsStay : d;
+    return (int)((packed >> (4 * d)) & 0xFu);
+}
+
+__device__ __forceinline__ uint32_t ps_lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int32_t ps_wave_scan(int32_t v) {
+    const int l = lane();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t w = __shfl_up(v, o);
+        if (l >= o) v += w;
+    }
+    return v;
+}
+
+// Particles of the workload's initial state: n0 per rank, positions uniform in the rank's
+// cell on a 2^-20 lattice, velocities on a 2^-15 lattice, id = rank * n0 + k.
+__global__ __launch_bounds__(kPsThreads) void ps_init_kernel(int gx, int gy, int gz, int64_t cap,
+                                                             int64_t n0, uint64_t seed, double* parts,
+                                                             int64_t* counts) {
+    const int P = gx * gy * gz;
+    const int64_t i = (int64_t)blockIdx.x * kPsThreads + threadIdx.x;
+    if (i < P) counts[i] = n0;
+    if (i >= (int64_t)P * n0) return;
+    const int rank = (int)(i / n0);
+    const int64_t k = i - (int64_t)rank * n0;
+    const int cx = rank % gx, cy = (rank / gx) % gy, cz = rank / (gx * gy);
+    const double q20 = 1.0 / 1048576.0, q15 = 1.0 / 32768.0;
+    const u32x4v o = rank_stream(seed, rank, 0u, (uint64_t)k, kStreamParticles);
+    double2* q = (double2*)(parts + ((uint64_t)rank * cap + k) * kPartFields);
+    q[0] = make_double2((double)cx + (double)(o.x >> 12) * q20, (double)cy + (double)(o.y >> 12) * q20);
+    q[1] = make_double2((double)cz + (double)(o.z >> 12) * q20,
+                        (double)((int32_t)(o.w & 0xFFFFu) - 32768) * q15);
+    q[2] = make_double2((double)((int32_t)(o.w >> 16) - 32768) * q15, (double)(k & 1023) * (1.0 / 1024.0));
+    q[3] = make_double2(1.0, (double)((int64_t)rank * n0 + k));
+}
+
+// Direction bytes of (in, cin) at `step`; resets the step counter and the statistics.
+__global__ __launch_bounds__(kPsThreads) void ps_dirs_kernel(PartStepDev a, uint32_t step) {
+    const int64_t i = (int64_t)blockIdx.x * kPsThreads + threadIdx.x;
+    if (i == 0) {
+        a.step[0] = step;
+        a.stats[0] = 0;
+        a.stats[1] = 0;
+        a.stats[2] = 0;
+    }
+    if (i >= (int64_t)a.P * a.cap) return;
+    const int r = (int)(i / a.cap);
+    const int64_t k = i - (int64_t)r * a.cap;
+    if (k >= ps_count_of(a, r)) return;
+    const int d = particle_dir(a.seed, step, (uint64_t)a.in[(uint64_t)i * kPartFields + 7]);
+    a.dir_out[i] = (uint8_t)(d < 0 ? kPsStay : d);
+}
+
+__global__ __launch_bounds__(kPsThreads) void ps_count_kernel(PartStepDev a) {
+    const int64_t w = ((int64_t)blockIdx.x * kPsThreads + threadIdx.x) >> 6;
+    if (w >= (int64_t)a.P * a.nchunks) return;
+    const int r = (int)(w / a.nchunks);
+    const int64_t k0 = (w - (int64_t)r * a.nchunks) * kPsChunk;
+    const int64_t n = ps_count_of(a, r);
+    const uint32_t cls = a.cls_tab[r];
+    const int l = lane();
+    const uint8_t* dir = a.dir_in + (uint64_t)r * a.cap;
+    int c[kPsChunk / kWave];
+#pragma unroll
+    for (int g = 0; g < kPsChunk / kWave; g++) {
+        const int64_t k = k0 + g * kWave + l;
+        c[g] = k < n ? ps_class_of(cls, dir[k]) : kPsSkip;
+    }
+    int32_t mine = 0;
+#pragma unroll
+    for (int x = 0; x <= kPsStay; x++) {
+        int32_t t = 0;
+#pragma unroll
+        for (int g = 0; g < kPsChunk / kWave; g++) t += __popcll(ballot(c[g] == x));
+        mine = l == x ? t : mine;
+    }
+    if (l < kPsClasses) a.chunk_off[w * kPsClasses + l] = mine;
+}
+
+__global__ __launch_bounds__(kPsThreads) void ps_scan_kernel(PartStepDev a) {
+    const int r = (int)(((int64_t)blockIdx.x * kPsThreads + threadIdx.x) >> 6);
+    if (r == 0 && lane() == 0) a.stats[1] = 0;
+    if (r >= a.P) return;
+    const int l = lane();
+    int32_t carry[kPsStay + 1];
+#pragma unroll
+    for (int x = 0; x <= kPsStay; x++) carry[x] = 0;
+    int32_t* row = a.chunk_off + (int64_t)r * a.nchunks * kPsClasses;
+    for (int c0 = 0; c0 < a.nchunks; c0 += kWave) {
+        const int c = c0 + l;
+        int4 v0{0, 0, 0, 0}, v1{0, 0, 0, 0};
+        if (c < a.nchunks) {
+            v0 = ((const int4*)(row + (int64_t)c * kPsClasses))[0];
+            v1 = ((const int4*)(row + (int64_t)c * kPsClasses))[1];
+        }
+        int32_t v[kPsClasses] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, 0};
+#pragma unroll
+        for (int x = 0; x <= kPsStay; x++) {
+            const int32_t incl = ps_wave_scan(v[x]);
+            const int32_t excl = incl - v[x] + carry[x];
+            carry[x] += __builtin_amdgcn_readlane(incl, 63);
+            v[x] = excl;
+        }
+        if (c < a.nchunks) {
+            ((int4*)(row + (int64_t)c * kPsClasses))[0] = int4{v[0], v[1], v[2], v[3]};
+            ((int4*)(row + (int64_t)c * kPsClasses))[1] = int4{v[4], v[5], v[6], 0};
+        }
+    }
+    int64_t t = 0;
+#pragma unroll
+    for (int x = 0; x <= kPsStay; x++) t = l == x ? carry[x] : t;
+    if (l < kPsClasses) a.tot[(int64_t)r * kPsClasses + l] = t;
+}
+
+// Destination j's slab after the step: its stayers, then for every source s that sends to
+// j, in ascending s, s's migrants toward j (one class of s reaches j).  The sources of j
+// are its own neighbours: grid_neighbour(s, d) == j exactly when s is j's neighbour in the
+// opposite direction.
+__global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
+    const int j = (int)((int64_t)blockIdx.x * kPsThreads + threadIdx.x);
+    if (j == 0) a.step[0] += 1u;
+    if (j >= a.P) return;
+    const int64_t stay = a.tot[(int64_t)j * kPsClasses + kPsStay];
+    a.base[(int64_t)j * kPsClasses + kPsStay] = 0;
+    int64_t acc = stay;
+    int prev = -1;
+    for (int t = 0; t < 6; t++) {
+        int s = a.P;   // the smallest neighbour of j above prev
+        for (int d = 0; d < 6; d++) {
+            const int nb = a.nbr_tab[(int64_t)j * kPsClasses + d];
+            s = nb > prev && nb < s ? nb : s;
+        }
+        if (s == a.P) break;
+        prev = s;
+        int cl = -1;
+        for (int d = 5; d >= 0; d--)
+            if (a.nbr_tab[(int64_t)s * kPsClasses + d] == j) cl = d;
+        a.base[(int64_t)s * kPsClasses + cl] = acc;
+        acc += a.tot[(int64_t)s * kPsClasses + cl];
+    }
+    a.cout[j] = acc;
+    if (acc > a.cap) {
+        a.stats[1] = 1;
+        a.stats[2] = 1;
+    }
+    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(acc - stay));
+}
+
+// Moves one chunk's particles.  The wavefront reads the chunk's 64 particles of a group as
+// four fully coalesced 1 KiB rows (lane l holds quarter l % 4 of particle 16 q + l / 4 in
+// row q); lane l also decides particle l of the group (its class, its rank within the class
+// by ballots, its final slot and its next direction), and the slots reach the lanes
+// holding the data by ds_bpermute.  Stores of consecutive particles to consecutive slots
+// form 1 KiB rows as well.
+template <bool NT>
+__global__ __launch_bounds__(kPsThreads) void ps_scatter_kernel(PartStepDev a) {
+    constexpr int G = kPsChunk / kWave;
+    const int64_t w = ((int64_t)blockIdx.x * kPsThreads + threadIdx.x) >> 6;
+    if (w >= (int64_t)a.P * a.nchunks) return;
+    const int r = (int)(w / a.nchunks);
+    const int64_t k0 = (w - (int64_t)r * a.nchunks) * kPsChunk;
+    const int64_t n = ps_count_of(a, r);
+    if (k0 >= n) return;
+    const int l = lane();
+
+    // every load of the chunk first, unconditionally: slots past the count read the
+    // chunk's first particle (in bounds) and are dropped below
+    const uint64_t slab = (uint64_t)r * a.cap;
+    const u32x4* src = (const u32x4*)(a.in + (slab + k0) * kPartFields);
+    u32x4 v[G][4];
+    uint64_t idbits[G];
+    uint32_t dirb[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int p = g * kWave + q * 16 + (l >> 2);
+            const int idx = k0 + p < n ? p * 4 + (l & 3) : (l & 3);
+            v[g][q] = src[idx];
+        }
+        const int64_t k = k0 + g * kWave + l < n ? k0 + g * kWave + l : k0;
+        idbits[g] = *(const uint64_t*)(a.in + (slab + k) * kPartFields + 7);
+        dirb[g] = a.dir_in[slab + k];
+    }
+
+    const uint32_t cls = a.cls_tab[r];
+    const bool dropped = a.stats[1] != 0;
+    const uint32_t next_step = a.step[0];
+    // slot cursors per class: destination slab start + group start + chunk offset
+    int64_t cur[kPsStay + 1];
+    {
+        const int x = l < kPsClasses ? l : 0;
+        const int dest = x < 6 ? a.nbr_tab[(int64_t)r * kPsClasses + x] : r;
+        int64_t cv = (int64_t)dest * a.cap + a.base[(int64_t)r * kPsClasses + x] +
+                     a.chunk_off[w * kPsClasses + x];
+#pragma unroll
+        for (int c = 0; c <= kPsStay; c++) cur[c] = (int64_t)readlane64((uint64_t)cv, c);
+    }
+
+    u32x4* dst = (u32x4*)a.out;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        // lane l: particle l of the group
+        const int d = k0 + g * kWave + l < n ? (dirb[g] > kPsStay ? kPsStay : (int)dirb[g]) : kPsSkip;
+        const int c = d == kPsSkip ? kPsSkip : ps_class_of(cls, d);
+        int64_t slot = -1;
+#pragma unroll
+        for (int x = 0; x <= kPsStay; x++) {
+            const uint64_t m = ballot(c == x);
+            if (c == x) slot = cur[x] + (int64_t)ps_lanes_below(m);
+            cur[x] += __popcll(m);
+        }
+        int code = d;
+        if (c == kPsSkip || (dropped && c != kPsStay)) code = kPsSkip;
+        if (code != kPsSkip) {
+            double idd;
+            __builtin_memcpy(&idd, &idbits[g], 8);
+            const int nd = particle_dir(a.seed, next_step, (uint64_t)idd);
+            a.dir_out[slot] = (uint8_t)(nd < 0 ? kPsStay : nd);
+        }
+        const uint32_t lo = (uint32_t)(uint64_t)slot;
+        const uint32_t hi = ((uint32_t)((uint64_t)slot >> 32) & 0x00FFFFFFu) | ((uint32_t)code << 24);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int from = (q * 16 + (l >> 2)) << 2;
+            const uint32_t plo = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)lo);
+            const uint32_t phi = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)hi);
+            const int pd = (int)(phi >> 24);
+            if (pd == kPsSkip) continue;
+            u32x4 val = v[g][q];
+            const int part = l & 3;
+            const int ax = pd >> 1;
+            // the moved coordinate: x, y in quarter 0, z in quarter 1
+            if (pd < kPsStay && ((part == 0 && ax < 2) || (part == 1 && ax == 2))) {
+                const bool second = part == 0 && ax == 1;
+                uint64_t bits = second ? ((uint64_t)val.w << 32 | val.z) : ((uint64_t)val.y << 32 | val.x);
+                double cval;
+                __builtin_memcpy(&cval, &bits, 8);
+                double cn = cval + ((pd & 1) ? -1.0 : 1.0);
+                const double dim = (double)(ax == 0 ? a.gx : ax == 1 ? a.gy : a.gz);
+                if (cn >= dim) cn -= dim;
+                if (cn < 0.0) cn += dim;
+                __builtin_memcpy(&bits, &cn, 8);
+                if (second) { val.z = (uint32_t)bits; val.w = (uint32_t)(bits >> 32); }
+                else { val.x = (uint32_t)bits; val.y = (uint32_t)(bits >> 32); }
+            }
+            const uint64_t ps = ((uint64_t)(phi & 0x00FFFFFFu) << 32) | plo;
+            u32x4* p = dst + ps * 4 + part;
+            if (NT) __builtin_nontemporal_store(val, p);
+            else *p = val;
+        }
+    }
+}
+
+}  // namespace gpumpi

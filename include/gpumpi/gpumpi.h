@@ -18,6 +18,8 @@
  *   gpumpi_last_error ...... CudaMPI::recordError / printLastError (gpu_libc/assert.cu:24-71)
  *   gpumpi_a2av_*  ......... no reference counterpart: the batched persistent-kernel
  *                            MPI_Alltoallv step (gpu_mpi/mpi.cu:363-408 semantics)
+ *   gpumpi_particles_* ..... no reference counterpart: the batched particle step (the
+ *                            particle variant of the uneven exchange)
  *   gpumpi_serve_*, gpumpi_run_job ... no reference counterpart: the device-resident job
  *                            server and the one-launch-per-job plain path
  *
@@ -154,6 +156,49 @@ int gpumpi_a2av_phase_times(gpumpi_a2av_plan* plan, int32_t enable, double* us);
 int gpumpi_a2av_get_tune(gpumpi_a2av_plan* plan, gpumpi_a2av_tune* tune, int32_t* grid);
 /* Total payload bytes moved by the last completed step (reads them back from the device). */
 int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* plan, int64_t* bytes);
+
+/* ------------------------------------------------------------------------------------
+ * Batched particle step: the particle variant of the exchange (count, scan, place and
+ * scatter kernels per step, hipGraph replay), with the result of the MPI program that
+ * exchanges the migrants by MPI_Alltoall of the counts + MPI_Alltoallv
+ * (gpu_mpi/mpi.cu:333-408 semantics).  No reference counterpart.
+ *
+ * P = gx*gy*gz ranks on a periodic grid; a state is parts[P][cap][8] FP64 {x, y, z, vx,
+ * vy, vz, w, id} and counts[P] (int64), device buffers.  Per step a particle moves to a
+ * face neighbour with probability ~0.05 by Philox(seed, id, step); each rank's new slab
+ * holds its stayers in order, then the arrivals in ascending source rank.  If any rank
+ * would receive more than cap particles, the step writes the stayers and the new counts
+ * only and sets the overflow flag (the MPI program's error path).
+ * The plan keeps a direction byte per slot for the state it will read next:
+ * gpumpi_particles_prepare computes them for a state at a given step, every step writes
+ * them for its output, so steps must follow each other on the same chain of buffers.
+ * ------------------------------------------------------------------------------------ */
+typedef struct gpumpi_particle_plan gpumpi_particle_plan;
+
+/* nontemporal: 1 = non-temporal stores in the scatter kernel. */
+int gpumpi_particles_plan_create(int32_t gx, int32_t gy, int32_t gz, int64_t cap, uint64_t seed,
+                                 int32_t nontemporal, gpumpi_particle_plan** out);
+void gpumpi_particles_plan_destroy(gpumpi_particle_plan* plan);
+/* The workload's initial state: n0 particles per rank (asynchronous on `stream`). */
+int gpumpi_particles_init(gpumpi_particle_plan* plan, double* parts, int64_t* counts, int64_t n0,
+                          void* stream);
+/* Direction bytes of (parts, counts) at `step`; resets the plan's statistics. */
+int gpumpi_particles_prepare(gpumpi_particle_plan* plan, const double* parts, const int64_t* counts,
+                             uint32_t step, void* stream);
+/* One step in -> out (asynchronous; in != out). */
+int gpumpi_particles_step(gpumpi_particle_plan* plan, const double* in, const int64_t* cin,
+                          double* out, int64_t* cout, void* stream);
+/* `steps` (even) steps a -> b -> a ..., captured into a hipGraph once, then replayed; the
+ * state ends in (a, ca).  The step index carries on across replays. */
+int gpumpi_particles_graph_replay(gpumpi_particle_plan* plan, double* a, int64_t* ca, double* b,
+                                  int64_t* cb, int32_t steps, void* stream);
+/* Synchronises the device; migrants moved since prepare, sticky overflow flag, and the
+ * step index of the next step. */
+int gpumpi_particles_status(gpumpi_particle_plan* plan, int64_t* moved, int32_t* overflow,
+                            uint32_t* next_step);
+/* enable = 1/0 arms/disarms HIP events around the kernels of gpumpi_particles_step;
+ * -1 reads us[4] = {count, scan, place, scatter} of the last timed step. */
+int gpumpi_particles_phase_times(gpumpi_particle_plan* plan, int32_t enable, double* us);
 
 /* ------------------------------------------------------------------------------------
  * Bucket-sort caller of the exchange (the NPB IS rank() shape,

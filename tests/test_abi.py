@@ -42,3 +42,13 @@ def test_mpi_header_datatype_order(pkg):
     body = re.search(r"enum MPI_Datatype \{(.*?)\};", src, re.S).group(1)
     names = [x.strip() for x in body.replace("\n", " ").split(",") if x.strip()]
     assert names == pkg.DATATYPES
+
+
+def test_particle_plan_rejects_bad_arguments(pkg):
+    """Argument checks of the batched particle step return before any device call."""
+    lib = pkg.load_library()
+    h = ctypes.c_void_p()
+    for dims, cap in [((0, 2, 2), 64), ((2, 2, 2), 0), ((2, 2, 2), 1 << 31)]:
+        assert lib.gpumpi_particles_plan_create(*dims, cap, 1, 1, ctypes.byref(h)) == pkg.E_ARG
+    assert lib.gpumpi_particles_step(None, None, None, None, None, None) == pkg.E_ARG
+    assert lib.gpumpi_particles_graph_replay(None, None, None, None, None, 2, None) == pkg.E_ARG
