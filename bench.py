@@ -276,7 +276,7 @@ def particle_exchange(pkg, torch, args):
     per_step = (m1 - m0) / ((reps + 1) * g)
     # kernel durations of plain steps (events around each kernel), averaged
     plan.phase_times(True)
-    kt = np.zeros(4)
+    kt = np.zeros(3)
     nk = 6
     for i in range(nk):
         x, cx, y, cy = (B, cb, A, ca) if i % 2 == 0 else (A, ca, B, cb)
@@ -286,15 +286,14 @@ def particle_exchange(pkg, torch, args):
     kt /= nk
     nparts = P * n0
     scatter_bytes = nparts * (2 * 64 + 2)
-    scatter_gbps = scatter_bytes / (kt[3] * 1e-6) / 1e9
+    scatter_gbps = scatter_bytes / (kt[2] * 1e-6) / 1e9
     plan.close()
     rg = out["alltoallv"]["ms_per_step"]
     out["batched"] = {
         "ms_per_step": round(ms, 4), "steps_timed": reps * g, "migrants_per_step": round(per_step),
         "GBps_migrants": round(2 * 64 * per_step / (ms * 1e-3) / 1e9, 2),
         "GBps_state": round(2 * 64 * nparts / (ms * 1e-3) / 1e9, 1),
-        "kernel_us": {"count": round(kt[0], 1), "scan": round(kt[1], 1), "place": round(kt[2], 1),
-                      "scatter": round(kt[3], 1)},
+        "kernel_us": {"tally": round(kt[0], 1), "place": round(kt[1], 1), "scatter": round(kt[2], 1)},
         "scatter_roofline": {"bound": "hbm", "achieved": round(scatter_gbps, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(scatter_gbps / HBM_PEAK_GBS, 4),
                              "bytes_per_launch": scatter_bytes},
@@ -302,8 +301,8 @@ def particle_exchange(pkg, torch, args):
         "vs_rank_grid_alltoallv": round(rg / ms, 2),
     }
     log(f"particles (batched): {ms:.3f} ms/step ({rg / ms:.1f}x the rank-grid Alltoallv step), "
-        f"scatter {kt[3]:.1f} us = {scatter_gbps:.0f} GB/s ({100 * scatter_gbps / HBM_PEAK_GBS:.1f} % of peak), "
-        f"count/scan/place {kt[0]:.1f}/{kt[1]:.1f}/{kt[2]:.1f} us; same state as the rank grid: {same}")
+        f"scatter {kt[2]:.1f} us = {scatter_gbps:.0f} GB/s ({100 * scatter_gbps / HBM_PEAK_GBS:.1f} % of peak), "
+        f"tally/place {kt[0]:.1f}/{kt[1]:.1f} us; same state as the rank grid: {same}")
     return out
 
 

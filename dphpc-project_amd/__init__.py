@@ -422,8 +422,8 @@ class A2avPlan:
 
 
 class ParticlePlan:
-    """Batched particle step on a periodic gx*gy*gz rank grid (gpumpi_particles_*): count,
-    scan, place and scatter kernels per step, replayed as one hipGraph.  States are device
+    """Batched particle step on a periodic gx*gy*gz rank grid (gpumpi_particles_*): tally,
+    place and scatter kernels per step, replayed as one hipGraph.  States are device
     buffers parts[P][cap][8] (float64) and counts[P] (int64)."""
 
     def __init__(self, dims, cap, seed, nontemporal=True):
@@ -468,11 +468,11 @@ class ParticlePlan:
 
     def phase_times(self, enable=None):
         """enable=True/False arms/disarms event timing of step(); with no argument returns
-        [count_us, scan_us, place_us, scatter_us] of the last timed step."""
+        [tally_us, place_us, scatter_us] of the last timed step."""
         if enable is not None:
             self.lib.gpumpi_particles_phase_times(self.h, 1 if enable else 0, None)
             return None
-        out = (ctypes.c_double * 4)()
+        out = (ctypes.c_double * 3)()
         self._check(self.lib.gpumpi_particles_phase_times(self.h, -1, out), "phase_times")
         return list(out)
 

@@ -1104,7 +1104,7 @@ struct gpumpi_particle_plan {
     const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
     int graph_cur = -1;       // direction buffer the captured steps start from
     bool timing = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 static int ps_launch(gpumpi_particle_plan* pl, const double* in, const int64_t* cin, double* out,
@@ -1117,20 +1117,20 @@ static int ps_launch(gpumpi_particle_plan* pl, const double* in, const int64_t* 
     d.cout = cout;
     d.dir_out = pl->dirs[pl->cur ^ 1];
     const int64_t waves = (int64_t)d.P * d.nchunks;
-    const dim3 chunk_grid((unsigned)((waves + 3) / 4)), rank_grid((unsigned)((d.P + 3) / 4)),
+    const dim3 chunk_grid((unsigned)((waves + 3) / 4)), rank_grid((unsigned)d.P),
         place_grid((unsigned)((d.P + kPsThreads - 1) / kPsThreads)), block(kPsThreads);
     void* kargs[] = {(void*)&d};
     const void* scatter = pl->nontemporal ? (const void*)&ps_scatter_kernel<true>
                                           : (const void*)&ps_scatter_kernel<false>;
-    const void* kernels[4] = {(const void*)&ps_count_kernel, (const void*)&ps_scan_kernel,
-                              (const void*)&ps_place_kernel, scatter};
-    const dim3 grids[4] = {chunk_grid, rank_grid, place_grid, chunk_grid};
-    for (int i = 0; i < 4; i++) {
+    const void* kernels[3] = {(const void*)&ps_tally_kernel, (const void*)&ps_place_kernel, scatter};
+    const dim3 grids[3] = {rank_grid, place_grid, chunk_grid};
+    const dim3 blocks[3] = {dim3(kPsTallyThreads), block, block};
+    for (int i = 0; i < 3; i++) {
         if (timed) hipEventRecord(pl->ev[i], s);
-        if (hipLaunchKernel(kernels[i], grids[i], block, kargs, 0, s) != hipSuccess)
+        if (hipLaunchKernel(kernels[i], grids[i], blocks[i], kargs, 0, s) != hipSuccess)
             return GPUMPI_E_HIP;
     }
-    if (timed) hipEventRecord(pl->ev[4], s);
+    if (timed) hipEventRecord(pl->ev[3], s);
     pl->cur ^= 1;
     return GPUMPI_OK;
 }
@@ -1172,7 +1172,7 @@ int gpumpi_particles_plan_create(int32_t gx, int32_t gy, int32_t gz, int64_t cap
         nbr[(size_t)r * kPsClasses + 7] = r;
         cls[r] = ps_classes(r, gx, gy, gz);
     }
-    const size_t b_dirs = (size_t)P * (size_t)cap, b_chunk = sizeof(int32_t) * (size_t)P * nchunks * kPsClasses,
+    const size_t b_dirs = (size_t)P * (size_t)cap + 256, b_chunk = sizeof(int32_t) * (size_t)P * nchunks * kPsClasses,
                  b_rank = sizeof(int64_t) * (size_t)P * kPsClasses, b_nbr = nbr.size() * sizeof(int32_t),
                  b_cls = cls.size() * sizeof(uint32_t);
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -1196,7 +1196,7 @@ int gpumpi_particles_plan_create(int32_t gx, int32_t gy, int32_t gz, int64_t cap
     hipMemcpy(q, cls.data(), b_cls, hipMemcpyHostToDevice); q += al(b_cls);
     d.stats = (int64_t*)q;
     d.step = (uint32_t*)(q + 4 * sizeof(int64_t));
-    for (int i = 0; i < 5; i++) hipEventCreate(&pl->ev[i]);
+    for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
     if (hipDeviceSynchronize() != hipSuccess) {
         gpumpi_particles_plan_destroy(pl);
         return GPUMPI_E_HIP;
@@ -1304,8 +1304,8 @@ int gpumpi_particles_phase_times(gpumpi_particle_plan* pl, int32_t enable, doubl
         return GPUMPI_OK;
     }
     if (!pl->timing || !us) return GPUMPI_E_ARG;
-    if (hipEventSynchronize(pl->ev[4]) != hipSuccess) return GPUMPI_E_HIP;
-    for (int i = 0; i < 4; i++) {
+    if (hipEventSynchronize(pl->ev[3]) != hipSuccess) return GPUMPI_E_HIP;
+    for (int i = 0; i < 3; i++) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, pl->ev[i], pl->ev[i + 1]) != hipSuccess) return GPUMPI_E_HIP;
         us[i] = 1000.0 * ms;

@@ -12,10 +12,9 @@
 // +-1 and wrapped.
 //
 // One step, reading (in, cin, dir_in) and writing (out, cout, dir_out):
-//   ps_count_kernel    one wavefront per chunk of kPsChunk slots: counts per class (the
-//                      six directions merged where two reach the same rank, and stay)
-//   ps_scan_kernel     one wavefront per rank: exclusive offsets of every chunk within the
-//                      rank, per class, and the rank's totals
+//   ps_tally_kernel    one workgroup per rank: counts per class of every chunk of kPsChunk
+//                      slots (the six directions merged where two reach the same rank, and
+//                      stay), their exclusive offsets within the rank, the rank's totals
 //   ps_place_kernel    one thread per destination rank: the first slot of every arriving
 //                      (source, class) group, the new counts, the overflow flag, the step
 //   ps_scatter_kernel  one wavefront per chunk: moves every particle to its final slot,
@@ -44,6 +43,8 @@ constexpr int kPsThreads = 256;   // four chunks per workgroup
 constexpr int kPsStay = 6;        // direction byte of a particle that stays
 constexpr int kPsSkip = 7;        // internal: slot beyond the count, or a dropped migrant
 constexpr int kPsClasses = 8;     // per chunk / rank: six directions, stay, padding
+constexpr int kPsTallyThreads = 256; // tally workgroup: one per rank
+constexpr int kPsTallyBatch = 12;    // 16-byte reads in flight per thread while counting
 
 __host__ __device__ __forceinline__ int grid_neighbour(int r, int d, int gx, int gy, int gz) {
     int x = r % gx, y = (r / gx) % gy, z = r / (gx * gy);
@@ -165,65 +166,100 @@ __global__ __launch_bounds__(kPsThreads) void ps_dirs_kernel(PartStepDev a, uint
     a.dir_out[i] = (uint8_t)(d < 0 ? kPsStay : d);
 }
 
-__global__ __launch_bounds__(kPsThreads) void ps_count_kernel(PartStepDev a) {
-    const int64_t w = ((int64_t)blockIdx.x * kPsThreads + threadIdx.x) >> 6;
-    if (w >= (int64_t)a.P * a.nchunks) return;
-    const int r = (int)(w / a.nchunks);
-    const int64_t k0 = (w - (int64_t)r * a.nchunks) * kPsChunk;
+typedef u32x4 ps_u32x4_a1 __attribute__((aligned(1)));   // a slab may start at any byte
+
+// One workgroup per rank: counts and exclusive offsets of its chunks per class, and the
+// rank's totals.
+//  count: thread t reads 16 direction bytes per round (slots 4096 j + 16 t .. +15, so the
+//         16 threads 16 m .. 16 m + 15 cover chunk 16 j + m), kPsTallyBatch rounds in flight,
+//         and adds one per slot into 9-bit fields of a 64-bit word (one field per class);
+//         the 16 threads of a chunk sum their words and store the chunk's counts;
+//  scan:  after a barrier, thread t takes chunk c0 + t of each pass of 256 chunks: wave
+//         scans, wave totals through LDS, a running carry between passes.
+__global__ __launch_bounds__(kPsTallyThreads) void ps_tally_kernel(PartStepDev a) {
+    const int r = blockIdx.x;
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+    if (r == 0 && t == 0) a.stats[1] = 0;
     const int64_t n = ps_count_of(a, r);
     const uint32_t cls = a.cls_tab[r];
-    const int l = lane();
     const uint8_t* dir = a.dir_in + (uint64_t)r * a.cap;
-    int c[kPsChunk / kWave];
+    int32_t* row = a.chunk_off + (int64_t)r * a.nchunks * kPsClasses;
+    constexpr int kChunksPerRound = 16 * kPsTallyThreads / kPsChunk;
+    const int rounds = (a.nchunks + kChunksPerRound - 1) / kChunksPerRound;
+    for (int j0 = 0; j0 < rounds; j0 += kPsTallyBatch) {
+        u32x4 word[kPsTallyBatch];
+        int valid[kPsTallyBatch];
 #pragma unroll
-    for (int g = 0; g < kPsChunk / kWave; g++) {
-        const int64_t k = k0 + g * kWave + l;
-        c[g] = k < n ? ps_class_of(cls, dir[k]) : kPsSkip;
+        for (int i = 0; i < kPsTallyBatch; i++) {
+            const int64_t k = (int64_t)(j0 + i) * 16 * kPsTallyThreads + 16 * t;
+            valid[i] = k < n ? (n - k < 16 ? (int)(n - k) : 16) : 0;
+            word[i] = valid[i] > 0 ? *(const ps_u32x4_a1*)(dir + k) : u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int i = 0; i < kPsTallyBatch; i++) {
+            uint64_t acc = 0;
+#pragma unroll
+            for (int b = 0; b < 16; b++) {
+                const uint32_t wd = b < 4 ? word[i].x : b < 8 ? word[i].y : b < 12 ? word[i].z : word[i].w;
+                const int c = ps_class_of(cls, (int)((wd >> (8 * (b & 3))) & 0xFFu));
+                acc += b < valid[i] ? 1ull << (9 * c) : 0ull;
+            }
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) acc += __shfl_xor(acc, o);
+            const int c = (j0 + i) * kChunksPerRound + (t >> 4);
+            const int x = l & 15;
+            if (x < kPsClasses && c < a.nchunks)
+                row[(int64_t)c * kPsClasses + x] = x <= kPsStay ? (int32_t)((acc >> (9 * x)) & 511u) : 0;
+        }
     }
-    int32_t mine = 0;
-#pragma unroll
-    for (int x = 0; x <= kPsStay; x++) {
-        int32_t t = 0;
-#pragma unroll
-        for (int g = 0; g < kPsChunk / kWave; g++) t += __popcll(ballot(c[g] == x));
-        mine = l == x ? t : mine;
-    }
-    if (l < kPsClasses) a.chunk_off[w * kPsClasses + l] = mine;
-}
+    __syncthreads();   // the chunk counts above are read by other threads of the workgroup
 
-__global__ __launch_bounds__(kPsThreads) void ps_scan_kernel(PartStepDev a) {
-    const int r = (int)(((int64_t)blockIdx.x * kPsThreads + threadIdx.x) >> 6);
-    if (r == 0 && lane() == 0) a.stats[1] = 0;
-    if (r >= a.P) return;
-    const int l = lane();
+    constexpr int kWaves = kPsTallyThreads / kWave;
+    __shared__ int32_t s_w[kWaves][kPsClasses];     // wave totals
+    __shared__ int32_t s_pre[kWaves][kPsClasses];   // exclusive prefix over the waves
+    __shared__ int32_t s_all[kPsClasses];
     int32_t carry[kPsStay + 1];
 #pragma unroll
     for (int x = 0; x <= kPsStay; x++) carry[x] = 0;
-    int32_t* row = a.chunk_off + (int64_t)r * a.nchunks * kPsClasses;
-    for (int c0 = 0; c0 < a.nchunks; c0 += kWave) {
-        const int c = c0 + l;
+    for (int c0 = 0; c0 < a.nchunks; c0 += kPsTallyThreads) {
+        const int c = c0 + t;
         int4 v0{0, 0, 0, 0}, v1{0, 0, 0, 0};
         if (c < a.nchunks) {
             v0 = ((const int4*)(row + (int64_t)c * kPsClasses))[0];
             v1 = ((const int4*)(row + (int64_t)c * kPsClasses))[1];
         }
-        int32_t v[kPsClasses] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, 0};
+        int32_t v[kPsStay + 1] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z};
+        int32_t incl[kPsStay + 1];
 #pragma unroll
         for (int x = 0; x <= kPsStay; x++) {
-            const int32_t incl = ps_wave_scan(v[x]);
-            const int32_t excl = incl - v[x] + carry[x];
-            carry[x] += __builtin_amdgcn_readlane(incl, 63);
-            v[x] = excl;
+            incl[x] = ps_wave_scan(v[x]);
+            if (l == kWave - 1) s_w[wv][x] = incl[x];
         }
+        __syncthreads();
+        if (t <= kPsStay) {
+            int32_t acc = 0;
+            for (int u = 0; u < kWaves; u++) {
+                s_pre[u][t] = acc;
+                acc += s_w[u][t];
+            }
+            s_all[t] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x <= kPsStay; x++) {
+            v[x] = incl[x] - v[x] + s_pre[wv][x] + carry[x];
+            carry[x] += s_all[x];
+        }
+        __syncthreads();
         if (c < a.nchunks) {
             ((int4*)(row + (int64_t)c * kPsClasses))[0] = int4{v[0], v[1], v[2], v[3]};
             ((int4*)(row + (int64_t)c * kPsClasses))[1] = int4{v[4], v[5], v[6], 0};
         }
     }
-    int64_t t = 0;
+    int64_t tt = 0;
 #pragma unroll
-    for (int x = 0; x <= kPsStay; x++) t = l == x ? carry[x] : t;
-    if (l < kPsClasses) a.tot[(int64_t)r * kPsClasses + l] = t;
+    for (int x = 0; x <= kPsStay; x++) tt = t == x ? carry[x] : tt;
+    if (t < kPsClasses) a.tot[(int64_t)r * kPsClasses + t] = tt;
 }
 
 // Destination j's slab after the step: its stayers, then for every source s that sends to
@@ -234,23 +270,44 @@ __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
     const int j = (int)((int64_t)blockIdx.x * kPsThreads + threadIdx.x);
     if (j == 0) a.step[0] += 1u;
     if (j >= a.P) return;
+    // round 1: j's neighbours (its possible sources) and its stayers
+    const int4* nt = (const int4*)a.nbr_tab;
+    const int4 n0 = nt[2 * j], n1 = nt[2 * j + 1];
     const int64_t stay = a.tot[(int64_t)j * kPsClasses + kPsStay];
+    int src[6] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y};
+    // ascending order (odd-even transposition sort; all indices fixed at compile time)
+#pragma unroll
+    for (int pass = 0; pass < 6; pass++) {
+#pragma unroll
+        for (int i = pass & 1; i + 1 < 6; i += 2) {
+            const int lo = src[i] < src[i + 1] ? src[i] : src[i + 1];
+            const int hi = src[i] < src[i + 1] ? src[i + 1] : src[i];
+            src[i] = lo;
+            src[i + 1] = hi;
+        }
+    }
+    // round 2: the class of each source toward j (the smallest direction reaching j)
+    int cl[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int4 m0 = nt[2 * src[i]], m1 = nt[2 * src[i] + 1];
+        const int m[6] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y};
+        int c = 0;
+#pragma unroll
+        for (int d = 5; d >= 0; d--) c = m[d] == j ? d : c;
+        cl[i] = c;
+    }
+    // round 3: the sources' counts toward j
+    int64_t cnt[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) cnt[i] = a.tot[(int64_t)src[i] * kPsClasses + cl[i]];
     a.base[(int64_t)j * kPsClasses + kPsStay] = 0;
     int64_t acc = stay;
-    int prev = -1;
-    for (int t = 0; t < 6; t++) {
-        int s = a.P;   // the smallest neighbour of j above prev
-        for (int d = 0; d < 6; d++) {
-            const int nb = a.nbr_tab[(int64_t)j * kPsClasses + d];
-            s = nb > prev && nb < s ? nb : s;
-        }
-        if (s == a.P) break;
-        prev = s;
-        int cl = -1;
-        for (int d = 5; d >= 0; d--)
-            if (a.nbr_tab[(int64_t)s * kPsClasses + d] == j) cl = d;
-        a.base[(int64_t)s * kPsClasses + cl] = acc;
-        acc += a.tot[(int64_t)s * kPsClasses + cl];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        if (i > 0 && src[i] == src[i - 1]) continue;   // a rank reached in two directions
+        a.base[(int64_t)src[i] * kPsClasses + cl[i]] = acc;
+        acc += cnt[i];
     }
     a.cout[j] = acc;
     if (acc > a.cap) {

@@ -158,8 +158,8 @@ int gpumpi_a2av_get_tune(gpumpi_a2av_plan* plan, gpumpi_a2av_tune* tune, int32_t
 int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* plan, int64_t* bytes);
 
 /* ------------------------------------------------------------------------------------
- * Batched particle step: the particle variant of the exchange (count, scan, place and
- * scatter kernels per step, hipGraph replay), with the result of the MPI program that
+ * Batched particle step: the particle variant of the exchange (tally, place and scatter
+ * kernels per step, hipGraph replay), with the result of the MPI program that
  * exchanges the migrants by MPI_Alltoall of the counts + MPI_Alltoallv
  * (gpu_mpi/mpi.cu:333-408 semantics).  No reference counterpart.
  *
@@ -197,7 +197,7 @@ int gpumpi_particles_graph_replay(gpumpi_particle_plan* plan, double* a, int64_t
 int gpumpi_particles_status(gpumpi_particle_plan* plan, int64_t* moved, int32_t* overflow,
                             uint32_t* next_step);
 /* enable = 1/0 arms/disarms HIP events around the kernels of gpumpi_particles_step;
- * -1 reads us[4] = {count, scan, place, scatter} of the last timed step. */
+ * -1 reads us[3] = {tally, place, scatter} of the last timed step. */
 int gpumpi_particles_phase_times(gpumpi_particle_plan* plan, int32_t enable, double* us);
 
 /* ------------------------------------------------------------------------------------

@@ -1,4 +1,4 @@
-"""Batched particle step (gpumpi_particles_*: count, scan, place, scatter kernels; hipGraph
+"""Batched particle step (gpumpi_particles_*: tally, place and scatter kernels; hipGraph
 replay) against the oracle's orc_particles_step, bit for bit: every FP64 field of every
 particle, the counts, the migrant totals and the overflow path.  At the workload's size
 (2^24 particles, 8x8x8 grid) against the rank-grid MPI program and by conservation."""
@@ -56,12 +56,14 @@ class Chain:
         return host_state(self.torch, self.bufs[self.cur], self.cnts[self.cur], self.P, self.cap)
 
 
-@pytest.mark.parametrize("dims", [(2, 2, 2), (4, 4, 4), (1, 2, 3), (1, 1, 1), (3, 1, 5), (2, 3, 1)])
-def test_batched_steps_match_oracle(pkg, oracle, torch_gpu, dims):
+@pytest.mark.parametrize("dims,cap", [((2, 2, 2), 2000), ((4, 4, 4), 2000), ((1, 2, 3), 2000),
+                                      ((1, 1, 1), 2000), ((3, 1, 5), 1001), ((2, 3, 1), 1203)])
+def test_batched_steps_match_oracle(pkg, oracle, torch_gpu, dims, cap):
     """Every step equals the oracle's, including grids whose axes of length 1 or 2 send
-    several directions to one rank or back to the sender."""
-    n0, cap, steps = 1000, 2000, 4   # cap not a multiple of the 256-slot chunk
-    print(f"[gpu] batched particles dims={dims}", flush=True)
+    several directions to one rank or back to the sender, and slabs whose size is not a
+    multiple of the 256-slot chunk or of 4 (direction bytes read at any alignment)."""
+    n0, steps = min(1000, cap * 4 // 5), 4
+    print(f"[gpu] batched particles dims={dims} cap={cap}", flush=True)
     parts, counts = particles_init(oracle, dims, n0, SEED, cap)
     ch = Chain(pkg, torch_gpu, dims, cap)
     ch.load(parts, counts, 0)
