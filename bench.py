@@ -288,6 +288,13 @@ def particle_exchange(pkg, torch, args):
     scatter_bytes = nparts * (2 * 64 + 2)
     scatter_gbps = scatter_bytes / (kt[2] * 1e-6) / 1e9
     plan.close()
+    scatter_traffic = None
+    tf = os.path.join(ROOT, "profiles", "ps_scatter_kernel_p512_2e24_traffic.json")
+    if os.path.exists(tf):
+        try:
+            scatter_traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            scatter_traffic = None
     rg = out["alltoallv"]["ms_per_step"]
     out["batched"] = {
         "ms_per_step": round(ms, 4), "steps_timed": reps * g, "migrants_per_step": round(per_step),
@@ -296,7 +303,7 @@ def particle_exchange(pkg, torch, args):
         "kernel_us": {"tally": round(kt[0], 1), "place": round(kt[1], 1), "scatter": round(kt[2], 1)},
         "scatter_roofline": {"bound": "hbm", "achieved": round(scatter_gbps, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(scatter_gbps / HBM_PEAK_GBS, 4),
-                             "bytes_per_launch": scatter_bytes},
+                             "bytes_per_launch": scatter_bytes, "traffic": scatter_traffic},
         "matches_rank_grid_11_steps": same,
         "vs_rank_grid_alltoallv": round(rg / ms, 2),
     }
