@@ -316,6 +316,88 @@ def particle_exchange(pkg, torch, args):
     return out
 
 
+def p2p_sweep(pkg, torch, args):
+    """Point-to-point shapes on 256 wavefront ranks (the reference's
+    benchmarks/mpi_communication/mpi_communication.c): ping-pong between ranks 2i and 2i+1
+    and a ring shift, each blocking (Send/Recv) and non-blocking (Isend/Irecv + Wait(all)),
+    seeded uint8 payloads of 8 B .. 64 MiB.  One-way time = round trip / 2 (ping-pong) or
+    the shift (ring), on the slowest rank; GB/s = bytes of all messages in flight per
+    one-way time (128 pairs, or 256 ring messages).  The program checks every received byte
+    against the sender's Philox payload: mismatches must be 0."""
+    from dphpc_project_amd import programs as pr
+    P = 256
+    sizes = [8 << (2 * i) for i in range(12)] + [64 << 20]
+    maxb = max(sizes)
+    sb = torch.zeros(P * maxb, dtype=torch.uint8, device="cuda")
+    rb = torch.zeros_like(sb)
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    mis = torch.zeros(P, dtype=torch.int64, device="cuda")
+    names = ("pingpong_blocking", "pingpong_nonblocking", "ring_blocking", "ring_nonblocking")
+    out = {"ranks": P, "bytes": sizes}
+    with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=60.0) as rt:
+        for mode, name in enumerate(names):
+            us, gbs = [], []
+            for b in sizes:
+                iters = 200 if b <= (64 << 10) else (40 if b <= (4 << 20) else 6)
+                mis.fill_(-1)
+                rt.run("p2p", pr.P2pArgs(0x77, mode, iters, 2, 0, b, sb.data_ptr(), rb.data_ptr(),
+                                         sec.data_ptr(), mis.data_ptr()))
+                torch.cuda.synchronize()
+                bad = int(mis.sum().item())
+                assert bad == 0, f"p2p {name} {b} B: {bad} mismatching bytes"
+                one_way = float(sec.max().item()) / iters / (2 if mode < 2 else 1)
+                msgs = P // 2 if mode < 2 else P
+                us.append(round(one_way * 1e6, 3))
+                gbs.append(round(msgs * b / one_way / 1e9, 2))
+            out[name] = {"us": us, "GBps": gbs}
+            log(f"p2p {name}: 8 B {us[0]:.2f} us, 64 KiB {us[6]:.2f} us, 64 MiB {us[-1]:.1f} us "
+                f"({gbs[-1]:.0f} GB/s)")
+    del sb, rb
+    torch.cuda.empty_cache()
+    return out
+
+
+def coll_sweep(pkg, torch, args):
+    """Collectives on 1024 wavefront ranks: Bcast, Reduce and Allreduce (MPI_SUM) and
+    Alltoall, on MPI_INT (uniform in [-2^20, 2^20]) and MPI_DOUBLE (N(0,1)), 1 .. 2^22
+    elements per rank (powers of 16, and 2^22; Alltoall: per-rank total).  Time per call
+    on the slowest rank; GB/s = algorithmic bytes per call: Bcast P*N*s, Reduce
+    (P+1)*N*s, Allreduce 2*P*N*s, Alltoall 2*P*N*s (N rounded down to whole P-element
+    rows: below P elements an Alltoall moves nothing)."""
+    from dphpc_project_amd import programs as pr
+    P = 1024
+    counts = [1 << (4 * i) for i in range(6)] + [1 << 22]
+    kinds = ("bcast", "reduce", "allreduce", "alltoall")
+    factor = {"bcast": P, "reduce": P + 1, "allreduce": 2 * P, "alltoall": 2 * P}
+    out = {"ranks": P, "counts": counts}
+    maxc = max(counts)
+    inb = torch.zeros(P * maxc * 8 + 16, dtype=torch.uint8, device="cuda")
+    outb = torch.zeros_like(inb)
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=128 << 20, timeout_s=60.0) as rt:
+        for dtype, dname, es in ((2, "MPI_INT", 4), (13, "MPI_DOUBLE", 8)):
+            res = {}
+            for kind in kinds:
+                us, gbs = [], []
+                for n in counts:
+                    iters = 50 if n <= 4096 else (10 if n <= (1 << 16) else 3)
+                    a = pr.CollArgs(0x1234, pr.COLL_KINDS[kind], dtype, 0, 0, n, iters, 1,
+                                    inb.data_ptr(), outb.data_ptr(), n * es, n * es, sec.data_ptr())
+                    rt.run("coll", a)
+                    torch.cuda.synchronize()
+                    t = float(sec.max().item()) / iters
+                    moved = (n // P) * P if kind == "alltoall" else n   # Alltoall: n // P per pair
+                    us.append(round(t * 1e6, 2))
+                    gbs.append(round(factor[kind] * moved * es / t / 1e9, 2))
+                res[kind] = {"us": us, "GBps": gbs}
+                log(f"{dname} {kind} on {P} ranks: 1 elem {us[0]:.1f} us, 2^22 elems {us[-1]:.0f} us "
+                    f"({gbs[-1]:.0f} GB/s)")
+            out[dname] = res
+    del inb, outb
+    torch.cuda.empty_cache()
+    return out
+
+
 def serve_latency(pkg, torch, args):
     """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
     the host, for one plain rank-grid launch per job and for the device-resident server.
@@ -445,6 +527,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-particles", action="store_true")
+    ap.add_argument("--no-npb", action="store_true")
+    ap.add_argument("--no-sweeps", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 2.0])
@@ -552,8 +636,20 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
 
+    npb = None
+    if not args.no_npb:
+        # NPB IS class C (2^27 keys < 2^23 on 512 ranks: this workload's size) through the
+        # batched exchange, checked by NPB's own partial and full verification
+        passed, ex_ms = pkg.npb_is("C", 512)
+        npb = {"class": "C", "ranks": 512, "passed_verification": passed, "expected": 5 * 10 + 512,
+               "exchange_ms_per_iteration": round(ex_ms / 10, 4)}
+        log(f"NPB IS class C on 512 ranks: passed_verification {passed} (expected 562), "
+            f"{ex_ms / 10:.3f} ms per iteration's exchange")
+        assert passed == 5 * 10 + 512, "NPB IS class C failed its verification"
     particles = None if args.no_particles else particle_exchange(pkg, torch, args)
     serve = None if args.no_serve else serve_latency(pkg, torch, args)
+    p2p = None if args.no_sweeps else p2p_sweep(pkg, torch, args)
+    colls = None if args.no_sweeps else coll_sweep(pkg, torch, args)
 
     cpu = None if args.no_cpu else cpu_baseline(args)
 
@@ -591,8 +687,11 @@ def main():
         "plain_path": plain,
         "batched_vs_plain": round(plain["ms_per_step"] / ms, 2) if plain else None,
         "zipf": zipf,
+        "npb_is": npb,
         "particles": particles,
         "serving": serve,
+        "p2p": p2p,
+        "collectives": colls,
     }
     print(json.dumps(result), flush=True)
 

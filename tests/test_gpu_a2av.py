@@ -220,10 +220,11 @@ def test_full_size_exchange(dist, pkg, oracle, torch_gpu):
 
 
 @pytest.mark.parametrize("cls,P", [("S", 1), ("S", 4), ("S", 16), ("S", 64), ("W", 8), ("W", 32),
-                                   ("A", 16)])
+                                   ("A", 16), ("B", 128), ("C", 512)])
 def test_npb_is_known_answers(cls, P, pkg, torch_gpu):
     """NPB IS partial + full verification through the batched exchange: 5*10 + P passes
-    (the class known answers of is.c:264-287 with the per-iteration shifts of 849-905)."""
+    (the class known answers of is.c:264-287 with the per-iteration shifts of 849-905).
+    Class C on 512 ranks is the exchange workload's size: 2^27 keys < 2^23, 2^18 per rank."""
     print(f"[gpu] NPB IS class {cls} on {P} ranks", flush=True)
     passed, ms = pkg.npb_is(cls, P)
     assert passed == 5 * 10 + P, passed
