@@ -21,7 +21,10 @@ served MPI jobs: Poisson arrivals of independent dot /
 Alltoall jobs of 8..256 ranks at fractions of the plain path's capacity, p50/p99
 latency (arrival -> result on the host) and jobs/s for the device-resident server
 versus one plain rank-grid launch per job; every served result is compared with the
-same job run alone on the plain path.
+same job run alone on the plain path.  Then the point-to-point sweep (256 ranks,
+ping-pong and ring, 8 B .. 64 MiB), the collective sweep (1024 ranks, 1 .. 2^22 elements)
+with the allocator churn, and NPB IS class C through the batched exchange as a known-answer
+cross-check at the workload's size.
 """
 import argparse
 import ctypes
@@ -320,13 +323,14 @@ def p2p_sweep(pkg, torch, args):
     """Point-to-point shapes on 256 wavefront ranks (the reference's
     benchmarks/mpi_communication/mpi_communication.c): ping-pong between ranks 2i and 2i+1
     and a ring shift, each blocking (Send/Recv) and non-blocking (Isend/Irecv + Wait(all)),
-    seeded uint8 payloads of 8 B .. 64 MiB.  One-way time = round trip / 2 (ping-pong) or
+    seeded uint8 payloads of 8 B .. 64 MiB (every power of two; 3 warm-ups, then 200
+    iterations up to 64 KiB and 20 above).  One-way time = round trip / 2 (ping-pong) or
     the shift (ring), on the slowest rank; GB/s = bytes of all messages in flight per
     one-way time (128 pairs, or 256 ring messages).  The program checks every received byte
     against the sender's Philox payload: mismatches must be 0."""
     from dphpc_project_amd import programs as pr
     P = 256
-    sizes = [8 << (2 * i) for i in range(12)] + [64 << 20]
+    sizes = [8 << i for i in range(24)]
     maxb = max(sizes)
     sb = torch.zeros(P * maxb, dtype=torch.uint8, device="cuda")
     rb = torch.zeros_like(sb)
@@ -338,9 +342,9 @@ def p2p_sweep(pkg, torch, args):
         for mode, name in enumerate(names):
             us, gbs = [], []
             for b in sizes:
-                iters = 200 if b <= (64 << 10) else (40 if b <= (4 << 20) else 6)
+                iters = 200 if b <= (64 << 10) else 20
                 mis.fill_(-1)
-                rt.run("p2p", pr.P2pArgs(0x77, mode, iters, 2, 0, b, sb.data_ptr(), rb.data_ptr(),
+                rt.run("p2p", pr.P2pArgs(0x77, mode, iters, 3, 0, b, sb.data_ptr(), rb.data_ptr(),
                                          sec.data_ptr(), mis.data_ptr()))
                 torch.cuda.synchronize()
                 bad = int(mis.sum().item())
@@ -350,7 +354,7 @@ def p2p_sweep(pkg, torch, args):
                 us.append(round(one_way * 1e6, 3))
                 gbs.append(round(msgs * b / one_way / 1e9, 2))
             out[name] = {"us": us, "GBps": gbs}
-            log(f"p2p {name}: 8 B {us[0]:.2f} us, 64 KiB {us[6]:.2f} us, 64 MiB {us[-1]:.1f} us "
+            log(f"p2p {name}: 8 B {us[0]:.2f} us, 64 KiB {us[13]:.2f} us, 64 MiB {us[-1]:.1f} us "
                 f"({gbs[-1]:.0f} GB/s)")
     del sb, rb
     torch.cuda.empty_cache()
@@ -358,16 +362,21 @@ def p2p_sweep(pkg, torch, args):
 
 
 def coll_sweep(pkg, torch, args):
-    """Collectives on 1024 wavefront ranks: Bcast, Reduce and Allreduce (MPI_SUM) and
-    Alltoall, on MPI_INT (uniform in [-2^20, 2^20]) and MPI_DOUBLE (N(0,1)), 1 .. 2^22
-    elements per rank (powers of 16, and 2^22; Alltoall: per-rank total).  Time per call
-    on the slowest rank; GB/s = algorithmic bytes per call: Bcast P*N*s, Reduce
-    (P+1)*N*s, Allreduce 2*P*N*s, Alltoall 2*P*N*s (N rounded down to whole P-element
-    rows: below P elements an Alltoall moves nothing)."""
+    """Collectives on 1024 wavefront ranks: Bcast, Reduce and Allreduce (MPI_SUM and
+    MPI_MAX) and Alltoall, on MPI_INT (uniform in [-2^20, 2^20]) and MPI_DOUBLE (N(0,1)),
+    1 .. 2^22 elements per rank (powers of 4; Alltoall: per-rank total).  Time per call on
+    the slowest rank; GB/s = algorithmic bytes per call: Bcast P*N*s, Reduce (P+1)*N*s,
+    Allreduce 2*P*N*s, Alltoall 2*P*N*s (N rounded down to whole P-element rows: below P
+    elements an Alltoall moves nothing).  Then the allocator churn of the same config:
+    per rank `--churn-ops` __gpu_malloc/__gpu_free of log-uniform 16 B .. 64 KiB blocks (at
+    most 64 live), an MPI_Allreduce every 1000 operations; heap operations per second over
+    all ranks, the heap's peak live bytes, and the stamp errors (must be 0)."""
     from dphpc_project_amd import programs as pr
     P = 1024
-    counts = [1 << (4 * i) for i in range(6)] + [1 << 22]
-    kinds = ("bcast", "reduce", "allreduce", "alltoall")
+    counts = [1 << (2 * i) for i in range(12)]
+    kinds = (("bcast", "bcast", 0), ("reduce_sum", "reduce", 0), ("reduce_max", "reduce", 1),
+             ("allreduce_sum", "allreduce", 0), ("allreduce_max", "allreduce", 1),
+             ("alltoall", "alltoall", 0))
     factor = {"bcast": P, "reduce": P + 1, "allreduce": 2 * P, "alltoall": 2 * P}
     out = {"ranks": P, "counts": counts}
     maxc = max(counts)
@@ -377,11 +386,11 @@ def coll_sweep(pkg, torch, args):
     with pkg.Runtime(P, heap_bytes=128 << 20, timeout_s=60.0) as rt:
         for dtype, dname, es in ((2, "MPI_INT", 4), (13, "MPI_DOUBLE", 8)):
             res = {}
-            for kind in kinds:
+            for name, kind, op in kinds:
                 us, gbs = [], []
                 for n in counts:
                     iters = 50 if n <= 4096 else (10 if n <= (1 << 16) else 3)
-                    a = pr.CollArgs(0x1234, pr.COLL_KINDS[kind], dtype, 0, 0, n, iters, 1,
+                    a = pr.CollArgs(0x1234, pr.COLL_KINDS[kind], dtype, op, 0, n, iters, 1,
                                     inb.data_ptr(), outb.data_ptr(), n * es, n * es, sec.data_ptr())
                     rt.run("coll", a)
                     torch.cuda.synchronize()
@@ -389,12 +398,31 @@ def coll_sweep(pkg, torch, args):
                     moved = (n // P) * P if kind == "alltoall" else n   # Alltoall: n // P per pair
                     us.append(round(t * 1e6, 2))
                     gbs.append(round(factor[kind] * moved * es / t / 1e9, 2))
-                res[kind] = {"us": us, "GBps": gbs}
-                log(f"{dname} {kind} on {P} ranks: 1 elem {us[0]:.1f} us, 2^22 elems {us[-1]:.0f} us "
+                res[name] = {"us": us, "GBps": gbs}
+                log(f"{dname} {name} on {P} ranks: 1 elem {us[0]:.1f} us, 2^22 elems {us[-1]:.0f} us "
                     f"({gbs[-1]:.0f} GB/s)")
             out[dname] = res
     del inb, outb
     torch.cuda.empty_cache()
+
+    ops = args.churn_ops
+    err = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    peak = torch.zeros(P, dtype=torch.int64, device="cuda")
+    lp = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    ls = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=4 << 30, timeout_s=120.0) as rt:
+        rt.run("churn", pr.ChurnArgs(0x5eed, ops, 1000, 0, err.data_ptr(), peak.data_ptr(),
+                                     lp.data_ptr(), ls.data_ptr(), sec.data_ptr()))
+        torch.cuda.synchronize()
+        stats = rt.heap_stats()
+    errors = int(err.sum().item())
+    assert errors == 0, f"heap churn: {errors} corrupted blocks"
+    t = float(sec.max().item())
+    out["churn"] = {"ranks": P, "ops_per_rank": ops, "seconds": round(t, 4),
+                    "heap_ops_per_s": round(P * ops / t, 1), "peak_live_bytes": stats["peak"],
+                    "in_use_after": stats["in_use"], "errors": errors, "allreduce_every": 1000}
+    log(f"heap churn: {P} ranks x {ops} ops in {t:.3f} s = {P * ops / t / 1e6:.1f} M ops/s, "
+        f"peak live {stats['peak'] / 2**20:.1f} MiB")
     return out
 
 
@@ -529,9 +557,10 @@ def main():
     ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--no-npb", action="store_true")
     ap.add_argument("--no-sweeps", action="store_true")
+    ap.add_argument("--churn-ops", type=int, default=1000000)
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=1024)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.5, 1.0, 2.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.25, 0.5, 1.0, 2.0, 4.0])
     ap.add_argument("--tune", type=str, default="",
                     help="block_threads,blocks_per_cu,lines_per_lane,nontemporal[,chunk_rounds[,defer_straddle]]")
     ap.add_argument("--copy-probe", action="store_true",

@@ -84,6 +84,7 @@ _SIGNATURES = {
     "gpumpi_run_kernel": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                          _P, ctypes.c_size_t, _P, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_main_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+    "gpumpi_layout_stamp": (ctypes.c_uint64, []),
     "gpumpi_last_error": (ctypes.c_char_p, [_P]),
     "gpumpi_last_device_error": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "gpumpi_program_count": (ctypes.c_int, []),

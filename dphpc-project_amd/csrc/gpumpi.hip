@@ -59,7 +59,8 @@ __global__ void init_runtime_kernel(Runtime* R, InitBases b) {
         s.initialized = 0;
         s.finalized = 0;
         s.arena = nullptr;
-        for (int c = 0; c < kHeapSmallClasses; c++) s.cache.n[c] = 0;
+        heap_cache_clear(&s.cache);
+        s.cache.shard = (int32_t)(r % kHeapShards);
         R->mailbox[r].tail = 0;
         R->world->slots[r].seq = 0;
         R->world->seqs[r] = 0;
@@ -71,7 +72,8 @@ __global__ void init_runtime_kernel(Runtime* R, InitBases b) {
         b.reqs[i].index = (int32_t)k;
     }
     for (uint64_t pg = tid; pg < R->heap.npages; pg += stride) R->heap.page_class[pg] = 0xFF;
-    if (tid < (uint64_t)kHeapClasses) R->heap.free_head[tid] = kHeapEmpty;
+    for (uint64_t i = tid; i < (uint64_t)kHeapClasses * kHeapShards; i += stride)
+        *heap_head(&R->heap, (int)(i / kHeapShards), (int)(i % kHeapShards)) = kHeapEmpty;
     if (tid == 0) {
         R->heap.page_bump = 0;
         R->heap.bytes_in_use = 0;
@@ -121,8 +123,9 @@ __global__ void resize_world_kernel(Runtime* R, int k) {
 __global__ void heap_reset_kernel(Runtime* R, int nstates) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (int r = tid; r < nstates; r += gridDim.x * blockDim.x)
-        for (int c = 0; c < kHeapSmallClasses; c++) R->ranks[r].cache.n[c] = 0;
-    if (tid < kHeapClasses) R->heap.free_head[tid] = kHeapEmpty;
+        heap_cache_clear(&R->ranks[r].cache);
+    for (int i = tid; i < kHeapClasses * kHeapShards; i += gridDim.x * blockDim.x)
+        *heap_head(&R->heap, i / kHeapShards, i % kHeapShards) = kHeapEmpty;
     if (tid == 0) {
         R->heap.page_bump = 0;
         R->heap.bytes_in_use = 0;
@@ -320,6 +323,7 @@ int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
     h.heap.bytes = heap;
     h.heap.npages = heap >> kHeapPageShift;
     ALLOC(h.heap.page_class, uint8_t, h.heap.npages)
+    ALLOC(h.heap.heads, uint64_t, (size_t)kHeapClasses * kHeapShards * kHeapHeadStride)
     // COMM_WORLD (reference: communicator.cu:52-62, context id 1) and the groups
     CommImpl world{};
     CollSlot* slots;
@@ -403,6 +407,8 @@ const char* gpumpi_last_error(gpumpi_runtime* rt) {
 }
 
 int gpumpi_program_count(void) { return kNumPrograms; }
+
+uint64_t gpumpi_layout_stamp(void) { return kLayoutStamp; }
 const char* gpumpi_program_name(int i) {
     return (i >= 0 && i < kNumPrograms) ? kPrograms[i].name : nullptr;
 }

@@ -44,7 +44,25 @@ __device__ void wave_copy_global(char* d, const char* s, uint64_t n, bool nt) {
     d += head; s += head; n -= head;
     const uint64_t nvec = n >> 4;
     const uint64_t sa = (uint64_t)(uintptr_t)s, dst = (uint64_t)(uintptr_t)d;
-    for (uint64_t base = 0; base < nvec; base += (uint64_t)kWave * U) {
+    uint64_t base = 0;
+    // bulk: whole rounds of kBulk vectors per lane, every load issued before the first
+    // store and none behind a bounds check (a lone wavefront's copy rate is set by the
+    // bytes it keeps in flight)
+    constexpr int kBulk = 8;
+    const uint64_t full = nvec - nvec % ((uint64_t)kWave * kBulk);
+    for (; base < full; base += (uint64_t)kWave * kBulk) {
+        u32x4 v[kBulk];
+#pragma unroll
+        for (int k = 0; k < kBulk; k++) v[k] = gload16(sa + 16 * (base + (uint64_t)k * kWave + l));
+        if (nt) {
+#pragma unroll
+            for (int k = 0; k < kBulk; k++) gstore16<true>(dst + 16 * (base + (uint64_t)k * kWave + l), v[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kBulk; k++) gstore16<false>(dst + 16 * (base + (uint64_t)k * kWave + l), v[k]);
+        }
+    }
+    for (; base < nvec; base += (uint64_t)kWave * U) {
         u32x4 v[U];
 #pragma unroll
         for (int k = 0; k < U; k++) {

@@ -13,6 +13,8 @@
 // system-scope CAS lock per receiver (gpu_libc/cuda_mpi.cuh:86-113, 372-390, 459-533).
 #pragma once
 
+#include <stddef.h>
+
 #include "common.h"
 #include "heap.h"
 #include "../../../include/gpumpi/io_ops.h"
@@ -175,5 +177,16 @@ struct Runtime {
     ErrorRecord err;
     IoSlot* io;           // nranks request slots in pinned host memory
 };
+
+// Layout of the structures shared by the library's host code and init kernels and by the
+// device code of a program compiled separately against these headers: a program built
+// against other headers than the library would read them at other offsets.  The launcher
+// main() compares its own value with the library's before it launches anything.
+constexpr uint64_t layout_mix(uint64_t h, uint64_t v) { return (h ^ v) * 0x100000001B3ull; }
+constexpr uint64_t kLayoutStamp =
+    layout_mix(layout_mix(layout_mix(layout_mix(layout_mix(layout_mix(layout_mix(
+        0xCBF29CE484222325ull, sizeof(Runtime)), sizeof(RankState)), sizeof(Heap)),
+        sizeof(HeapCache)), offsetof(Runtime, heap)), offsetof(RankState, cache)),
+        offsetof(Heap, bytes_in_use));
 
 }  // namespace gpumpi

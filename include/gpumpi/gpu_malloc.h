@@ -37,6 +37,8 @@ __device__ void __gpu_clean_malloc();
 __device__ void __gpu_init_malloc_v6();
 __device__ void __gpu_clean_malloc_v6();
 
-// Heap statistics of the running rank grid (bytes of live blocks, class-rounded).
+// Heap statistics of the running rank grid (bytes of live blocks, class-rounded).  A
+// running rank adds its own operations in batches (every 64 operations or 1 MiB net), so
+// while ranks run the value lags by at most that much per rank; it is exact once they end.
 __device__ long long __gpu_heap_in_use();
 __device__ long long __gpu_heap_capacity();

@@ -83,6 +83,10 @@ int gpumpi_run_kernel(gpumpi_runtime* rt, const void* kernel, int argc, const ch
                       const void* args, size_t args_bytes, void* stream, double* elapsed_ms);
 /* CLI of a compiled converted program: argv = "<exe> <args> ---gpumpi -n N ...". */
 int gpumpi_main_kernel(const void* kernel, int argc, const char* const* argv);
+/* Layout stamp of the runtime structures this library was built with (no reference
+ * counterpart): gpumpi_main.h compares it with the stamp of the headers a program was
+ * compiled against and refuses to launch on a mismatch (a stale build). */
+uint64_t gpumpi_layout_stamp(void);
 
 /* One-shot: create, run, destroy (what main.cu does for one converted program). */
 int gpumpi_launch(const char* program, int argc, const char* const* argv, const void* args,

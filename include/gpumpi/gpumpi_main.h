@@ -15,6 +15,8 @@
 #include "gpumpi.h"
 #include "runtime/rank_kernel.h"
 
+#include <stdio.h>
+
 __device__ int __gpu_main(int argc, char** argv);
 
 namespace gpumpi {
@@ -31,6 +33,11 @@ __device__ int gpumpi_checked_main(int argc, char** argv) {
 }  // namespace gpumpi
 
 int main(int argc, char** argv) {
+    if (gpumpi_layout_stamp() != gpumpi::kLayoutStamp) {
+        fprintf(stderr, "GPUMPI: this program was compiled against other runtime headers than "
+                        "the libgpumpi.so it loaded; rebuild it\n");
+        return 1;
+    }
     return gpumpi_main_kernel((const void*)&gpumpi::rank_grid_kernel<gpumpi::gpumpi_checked_main>,
                               argc, (const char* const*)argv);
 }

@@ -52,3 +52,19 @@ def test_particle_plan_rejects_bad_arguments(pkg):
         assert lib.gpumpi_particles_plan_create(*dims, cap, 1, 1, ctypes.byref(h)) == pkg.E_ARG
     assert lib.gpumpi_particles_step(None, None, None, None, None, None) == pkg.E_ARG
     assert lib.gpumpi_particles_graph_replay(None, None, None, None, None, 2, None) == pkg.E_ARG
+
+
+def test_built_programs_match_the_library_layout(pkg):
+    """A converted program compares the runtime layout it was compiled against with the
+    library's (gpumpi_layout_stamp) before it launches anything -- a stale build would read
+    the runtime's structures at other offsets.  `---gpumpi -h` passes the check and prints
+    the launcher's help without a GPU."""
+    import subprocess
+    import pytest
+    assert pkg.load_library().gpumpi_layout_stamp() != 0
+    exe = os.path.join(ROOT, "tests", "programs", "_build", "cpi")
+    if not os.path.exists(exe):
+        pytest.skip("converted programs not built")
+    r = subprocess.run([exe, "---gpumpi", "-h"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "GPU MPI" in r.stdout
