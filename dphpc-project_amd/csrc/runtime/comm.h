@@ -81,8 +81,8 @@ __device__ void coll_publish(CommImpl* c, int me, uint64_t seq, const void* sbuf
     s->a1 = a1;
     s->a2 = a2;
     s->a3 = a3;
-    fence_release();
-    if (lane() == 0) st_release(&s->seq, seq);
+    fence_release();  // with the fence, a relaxed store publishes (one L2 write-back, not two)
+    if (lane() == 0) st_relaxed(&s->seq, seq);
 }
 
 __device__ bool coll_wait_slot(CommImpl* c, int j, uint64_t seq) {

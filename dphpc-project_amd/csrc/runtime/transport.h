@@ -147,7 +147,7 @@ __device__ void deliver(Request* r, const EnvHeader& h, const Envelope* e) {
     } else {
         wave_copy(r->buf, (const void*)h.data, n);
         fence_release();
-        if (h.done && lane() == 0) st_release((uint32_t*)h.done, 1u);
+        if (h.done && lane() == 0) st_relaxed((uint32_t*)h.done, 1u);
         if (h.flags & kEnvBuffered) heap_free(&rt()->heap, &my_state()->cache, (void*)h.data);
     }
     r->st_source = h.src;
@@ -255,7 +255,7 @@ __device__ int progress() {
             rs->n_unexp = rs->n_unexp + 1;
         }
         fence_release();
-        if (lane() == 0) st_release(&e->seq, (uint32_t)(h + N));
+        if (lane() == 0) st_relaxed(&e->seq, (uint32_t)(h + N));
         h++;
         rs->ring_head = h;
         consumed++;
@@ -289,8 +289,8 @@ __device__ bool push_envelope(int dest, u32x4 v) {
         w0[2] = v.z;
         w0[3] = v.w;
     }
-    fence_release();
-    if (lane() == 0) st_release(&e->seq, (uint32_t)(t + 1));
+    fence_release();  // with the fence, a relaxed store publishes (one L2 write-back, not two)
+    if (lane() == 0) st_relaxed(&e->seq, (uint32_t)(t + 1));
     return true;
 }
 
