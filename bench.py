@@ -410,9 +410,10 @@ def coll_sweep(pkg, torch, args):
     peak = torch.zeros(P, dtype=torch.int64, device="cuda")
     lp = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
     ls = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    failed = torch.full((P,), -1, dtype=torch.int64, device="cuda")
     with pkg.Runtime(P, heap_bytes=4 << 30, timeout_s=120.0) as rt:
         rt.run("churn", pr.ChurnArgs(0x5eed, ops, 1000, 0, err.data_ptr(), peak.data_ptr(),
-                                     lp.data_ptr(), ls.data_ptr(), sec.data_ptr()))
+                                     lp.data_ptr(), ls.data_ptr(), sec.data_ptr(), failed.data_ptr()))
         torch.cuda.synchronize()
         stats = rt.heap_stats()
     errors = int(err.sum().item())
@@ -420,7 +421,8 @@ def coll_sweep(pkg, torch, args):
     t = float(sec.max().item())
     out["churn"] = {"ranks": P, "ops_per_rank": ops, "seconds": round(t, 4),
                     "heap_ops_per_s": round(P * ops / t, 1), "peak_live_bytes": stats["peak"],
-                    "in_use_after": stats["in_use"], "errors": errors, "allreduce_every": 1000}
+                    "in_use_after": stats["in_use"], "errors": errors,
+                    "failed_allocations": int(failed.sum().item()), "allreduce_every": 1000}
     log(f"heap churn: {P} ranks x {ops} ops in {t:.3f} s = {P * ops / t / 1e6:.1f} M ops/s, "
         f"peak live {stats['peak'] / 2**20:.1f} MiB")
     return out

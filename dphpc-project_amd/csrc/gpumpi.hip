@@ -60,7 +60,7 @@ __global__ void init_runtime_kernel(Runtime* R, InitBases b) {
         s.finalized = 0;
         s.arena = nullptr;
         heap_cache_clear(&s.cache);
-        s.cache.shard = (int32_t)(r % kHeapShards);
+        s.cache.shard = (int32_t)(r % (uint64_t)R->heap.nshards);
         R->mailbox[r].tail = 0;
         R->world->slots[r].seq = 0;
         R->world->seqs[r] = 0;
@@ -322,6 +322,7 @@ int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
     ALLOC(h.heap.base, char, heap)
     h.heap.bytes = heap;
     h.heap.npages = heap >> kHeapPageShift;
+    h.heap.nshards = heap_shard_count(h.heap.npages);
     ALLOC(h.heap.page_class, uint8_t, h.heap.npages)
     ALLOC(h.heap.heads, uint64_t, (size_t)kHeapClasses * kHeapShards * kHeapHeadStride)
     // COMM_WORLD (reference: communicator.cu:52-62, context id 1) and the groups

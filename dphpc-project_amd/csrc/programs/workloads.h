@@ -258,6 +258,7 @@ struct ChurnArgs {
     uint64_t* live_ptrs;   // [P * 64]
     uint64_t* live_sizes;  // [P * 64]
     double* seconds;       // [P]
+    int64_t* failed;       // [P] allocations that returned nullptr (arena exhausted); may be null
 };
 
 __device__ int w_churn(int, char**) {
@@ -269,7 +270,7 @@ __device__ int w_churn(int, char**) {
     uint64_t* ptrs = (uint64_t*)__gpu_malloc(sizeof(uint64_t) * 2 * kMaxLive);
     uint64_t* sizes = ptrs + kMaxLive;
     int live = 0;
-    int64_t errors = 0, peak = 0;
+    int64_t errors = 0, failed = 0, peak = 0;
     MPI_Barrier(MPI_COMM_WORLD);
     double t0 = MPI_Wtime();
     for (int64_t op = 0; op < a.ops; op++) {
@@ -292,7 +293,7 @@ __device__ int w_churn(int, char**) {
             if (sz < 8) sz = 8;
             void* p = __gpu_malloc(sz);
             if (!p) {
-                errors++;
+                failed++;
             } else {
                 uint64_t stamp = ((uint64_t)rank << 40) ^ sz;
                 *(uint64_t*)p = stamp;
@@ -318,7 +319,9 @@ __device__ int w_churn(int, char**) {
     MPI_Barrier(MPI_COMM_WORLD);
     for (int k = 0; k < live; k++) __gpu_free((void*)(uintptr_t)ptrs[k]);
     __gpu_free(ptrs);
-    a.errors[rank] = errors;
+    // without a `failed` array a failed allocation counts as an error
+    a.errors[rank] = a.failed ? errors : errors + failed;
+    if (a.failed) a.failed[rank] = failed;
     a.peak_live[rank] = peak;
     a.seconds[rank] = t1 - t0;
     MPI_Finalize();

@@ -12,11 +12,13 @@
 //   * power-of-two classes of 16 B << c; classes below 64 KiB share pages of one class
 //     (page_class[] maps a page to its class, so free() needs no block header);
 //     classes of 64 KiB and up take a run of whole pages;
-//   * per class, kHeapShards Treiber stacks whose 64-bit heads pack a 40-bit block offset
+//   * per class, nshards Treiber stacks whose 64-bit heads pack a 40-bit block offset
 //     (16 B units) with a 24-bit ABA tag; rank r pushes to and pops from shard
-//     r % kHeapShards and turns to the other shards only once the page bump is used up
-//     (one shared stack per class collapses under contention: with 1024 ranks a CAS
-//     round costs hundreds of microseconds);
+//     r % nshards and turns to the other shards only once the page bump is used up (one
+//     shared stack per class collapses under contention: with 1024 ranks a CAS round
+//     costs hundreds of microseconds).  nshards = one per 64 pages of the arena, at most
+//     kHeapShards: every shard holds a partly used page per small class, and on a small
+//     arena too many shards would pin the pages that 64 KiB blocks need;
 //   * a refill carves a fresh page, links its blocks with all 64 lanes and pushes the
 //     chain to the rank's shard with one CAS;
 //   * when the page bump is used up, a refill takes a free block of the smallest
@@ -43,7 +45,8 @@ constexpr int kHeapCacheDepth = 8;
 constexpr uint64_t kHeapOffMask = (1ull << 40) - 1;
 constexpr uint64_t kHeapEmpty = kHeapOffMask;  // offset value meaning "empty stack"
 constexpr int kHeapHeadStride = 16;            // uint64 words per stack head: 128 bytes
-constexpr int kHeapShards = 128;               // stacks per class
+constexpr int kHeapShards = 128;               // stacks per class at most
+constexpr uint64_t kHeapPagesPerShard = 64;
 constexpr int kHeapAcctOps = 64;               // statistics batch: operations ...
 constexpr int64_t kHeapAcctBytes = 1ll << 20;  // ... or net bytes of one rank
 
@@ -54,11 +57,18 @@ struct Heap {
     uint8_t* page_class;             // [npages], 0xFF = untouched
     uint64_t page_bump;              // atomic
     uint64_t* heads;                 // [kHeapClasses][kHeapShards][kHeapHeadStride]
+    int32_t nshards;                 // stacks in use per class (heap_shard_count)
+    int32_t pad;
     alignas(128) int64_t bytes_in_use;  // atomic: sum of class sizes of live blocks
     int64_t peak_in_use;             // atomic max (sampled when a rank adds its batch)
     int64_t alloc_count;             // atomic, statistics
     int64_t refill_count;            // atomic, statistics
 };
+
+__host__ __device__ __forceinline__ int32_t heap_shard_count(uint64_t npages) {
+    const uint64_t n = npages / kHeapPagesPerShard;
+    return (int32_t)(n < 1 ? 1 : n > (uint64_t)kHeapShards ? (uint64_t)kHeapShards : n);
+}
 
 __host__ __device__ __forceinline__ uint64_t* heap_head(Heap* h, int c, int shard) {
     return &h->heads[((size_t)c * kHeapShards + (size_t)shard) * kHeapHeadStride];
@@ -67,7 +77,7 @@ __host__ __device__ __forceinline__ uint64_t* heap_head(Heap* h, int c, int shar
 // Per-rank block cache and statistics batch (live in the rank's private state).
 struct HeapCache {
     int8_t n[kHeapSmallClasses];
-    int32_t shard;                   // the rank's stack shard: world rank % kHeapShards
+    int32_t shard;                   // the rank's stack shard: world rank % Heap::nshards
     int32_t acct_ops;                // operations since the batch was added
     int32_t pad;
     int64_t acct_bytes;              // net bytes allocated since then
@@ -128,8 +138,9 @@ __device__ __forceinline__ uint64_t heap_pop(Heap* h, int c, int shard) {
 
 // Lane-0 pop from the shard `home`, then from every other shard of class c.
 __device__ __forceinline__ uint64_t heap_pop_any(Heap* h, int c, int home) {
-    for (int k = 0; k < kHeapShards; k++) {
-        const uint64_t off = heap_pop(h, c, (home + k) % kHeapShards);
+    const int n = h->nshards;
+    for (int k = 0; k < n; k++) {
+        const uint64_t off = heap_pop(h, c, (home + k) % n);
         if (off != kHeapEmpty) return off;
     }
     return kHeapEmpty;

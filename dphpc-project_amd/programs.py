@@ -31,7 +31,7 @@ class CollArgs(ctypes.Structure):
 class ChurnArgs(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("ops", ctypes.c_int64), ("coll_every", ctypes.c_int32),
                 ("dump", ctypes.c_int32), ("errors", _P), ("peak_live", _P), ("live_ptrs", _P),
-                ("live_sizes", _P), ("seconds", _P)]
+                ("live_sizes", _P), ("seconds", _P), ("failed", _P)]
 
 
 class ExchArgs(ctypes.Structure):

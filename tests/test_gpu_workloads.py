@@ -146,6 +146,38 @@ def test_bcast_alltoall_gather_scatter(dtype, P, count, pkg, oracle, torch_gpu):
         assert np.array_equal(out[r], src[r * count:(r + 1) * count])
 
 
+def test_heap_churn_on_a_full_arena(pkg, oracle, torch_gpu):
+    """The churn on an arena too small for its live set (64 ranks, 32 MiB, ~26 MiB live at
+    peak): the page bump runs out, so allocations are served by other ranks' free stacks
+    and by splitting larger blocks, and some fail with nullptr.  No block may be handed out
+    twice (stamps intact, live blocks disjoint and aligned) and the heap ends empty."""
+    from dphpc_project_amd import programs as pr
+    from oracle_py import ptr
+    torch = torch_gpu
+    P, ops, heap = 64, 5000, 32 << 20
+    print(f"[gpu] heap churn on a full arena P={P} ops={ops} heap={heap >> 20} MiB", flush=True)
+    err = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    failed = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    peak = torch.zeros(P, dtype=torch.int64, device="cuda")
+    lp = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    ls = torch.zeros(P * 64, dtype=torch.int64, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=heap, timeout_s=60.0) as rt:
+        rt.run("churn", pr.ChurnArgs(0x5eed, ops, 100, 1, err.data_ptr(), peak.data_ptr(),
+                                     lp.data_ptr(), ls.data_ptr(), sec.data_ptr(), failed.data_ptr()))
+        stats = rt.heap_stats()
+    assert err.cpu().tolist() == [0] * P
+    assert int(failed.sum().item()) > 0, "the arena was expected to run out"
+    assert stats["in_use"] == 0
+    p = lp.cpu().numpy().view(np.uint64)
+    s = ls.cpu().numpy().view(np.uint64)
+    keep = p != 0
+    p, s = np.ascontiguousarray(p[keep]), np.ascontiguousarray(s[keep])
+    base = int(p.min()) & ~0xFFFF
+    assert oracle.orc_heap_check(ptr(p), ptr(s), ctypes.c_int64(p.size), ctypes.c_uint64(base),
+                                 ctypes.c_uint64(heap + (1 << 20))) == 0
+
+
 def test_heap_churn(pkg, oracle, torch_gpu):
     """Config 3 allocator churn at reduced length: stamps survive, live blocks are disjoint
     and aligned, the heap returns to empty, with an Allreduce interleaved."""
