@@ -122,6 +122,10 @@ __device__ __forceinline__ bool cas(T* p, T& expected, T desired) {
 // Orders this wave's payload stores before a following flag store (all lanes call it).
 __device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Waits until this wave's loads (and stores) have completed, without writing the L2 back:
+// enough before a store that only says "your data has been read" (the reads returned, so
+// no later store of another wave can change them) and publishes nothing of its own.
+__device__ __forceinline__ void wait_reads_done() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
 
 // ---------------------------------------------------------------- errors
 enum ErrorCode : int32_t {

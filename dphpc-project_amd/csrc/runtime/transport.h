@@ -254,7 +254,9 @@ __device__ int progress() {
             env_store(&rs->unexp[rs->n_unexp], v);
             rs->n_unexp = rs->n_unexp + 1;
         }
-        fence_release();
+        // the slot goes back to its producers once our reads of it are done; what we
+        // wrote (the receive buffer, the unexpected queue) is ours and needs no write-back
+        wait_reads_done();
         if (lane() == 0) st_relaxed(&e->seq, (uint32_t)(h + N));
         h++;
         rs->ring_head = h;
@@ -280,7 +282,8 @@ __device__ bool push_envelope(int dest, u32x4 v) {
         progress();  // drain our own ring so two full rings cannot deadlock
         if (!w.keep_going(__LINE__)) return false;
     }
-    fence_acquire();  // the consumer's reads of this slot happen before our writes
+    // no acquire fence: the consumer's reads of this slot completed before it handed the
+    // slot back (wait_reads_done in progress), and we only write the slot
     // lane 0 must not overwrite seq with the plain store; it writes words 1..3 only
     if (lane() >= 1 && lane() < 8) ((u32x4*)e)[lane()] = v;
     if (lane() == 0) {
