@@ -265,8 +265,12 @@ __device__ int progress() {
     return consumed;
 }
 
-// Publish an envelope (lanes 0..7 of v) in dest's ring.
-__device__ bool push_envelope(int dest, u32x4 v) {
+// Publish an envelope (lanes 0..7 of v) in dest's ring.  `self_contained`: the envelope
+// carries everything the receiver will read (an inline payload), so it is written with
+// agent-scope atomic stores, which reach the other XCDs without an L2 write-back, and only
+// their completion is awaited; otherwise the receiver will also read memory this wave
+// wrote before (the payload it pulls), and a release fence writes the L2 back first.
+__device__ bool push_envelope(int dest, u32x4 v, bool self_contained) {
     Runtime* R = rt();
     const uint32_t N = (uint32_t)R->ring_slots;
     uint64_t t = 0;
@@ -285,14 +289,27 @@ __device__ bool push_envelope(int dest, u32x4 v) {
     // no acquire fence: the consumer's reads of this slot completed before it handed the
     // slot back (wait_reads_done in progress), and we only write the slot
     // lane 0 must not overwrite seq with the plain store; it writes words 1..3 only
-    if (lane() >= 1 && lane() < 8) ((u32x4*)e)[lane()] = v;
-    if (lane() == 0) {
-        volatile uint32_t* w0 = (volatile uint32_t*)e;
-        w0[1] = v.y;
-        w0[2] = v.z;
-        w0[3] = v.w;
+    if (self_contained) {
+        if (lane() >= 1 && lane() < 8) {
+            uint64_t* q = (uint64_t*)((u32x4*)e + lane());
+            st_relaxed(q, ((uint64_t)v.y << 32) | v.x);
+            st_relaxed(q + 1, ((uint64_t)v.w << 32) | v.z);
+        }
+        if (lane() == 0) {
+            st_relaxed((uint32_t*)e + 1, v.y);
+            st_relaxed((uint64_t*)e + 1, ((uint64_t)v.w << 32) | v.z);
+        }
+        wait_reads_done();  // the stores above have completed
+    } else {
+        if (lane() >= 1 && lane() < 8) ((u32x4*)e)[lane()] = v;
+        if (lane() == 0) {
+            volatile uint32_t* w0 = (volatile uint32_t*)e;
+            w0[1] = v.y;
+            w0[2] = v.z;
+            w0[3] = v.w;
+        }
+        fence_release();  // with the fence, a relaxed store publishes (one L2 write-back, not two)
     }
-    fence_release();  // with the fence, a relaxed store publishes (one L2 write-back, not two)
     if (lane() == 0) st_relaxed(&e->seq, (uint32_t)(t + 1));
     return true;
 }
@@ -352,7 +369,7 @@ __device__ Request* isend(const void* buf, uint64_t bytes, int dest_world, int t
     if (l == 1) { v.x = (uint32_t)bytes; v.y = (uint32_t)(bytes >> 32);
                   v.z = (uint32_t)data; v.w = (uint32_t)(data >> 32); }
     if (l == 2) { v.x = (uint32_t)done; v.y = (uint32_t)(done >> 32); v.z = flags; v.w = 0u; }
-    if (!push_envelope(dest_world, v)) return r;
+    if (!push_envelope(dest_world, v, flags == kEnvInline)) return r;
     return r;
 }
 
