@@ -73,6 +73,19 @@ __device__ __forceinline__ void reduce_slice(int64_t count, int P, int k, int64_
     if (*hi > count) *hi = count;
 }
 
+// Value of `v` in lane j (uniform j), for any type of at most 8 bytes.
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int j) {
+    uint64_t b = 0;
+    __builtin_memcpy(&b, &v, sizeof(T));
+    b = readlane64(b, j);
+    T out;
+    __builtin_memcpy(&out, &b, sizeof(T));
+    return out;
+}
+
+constexpr int64_t kReduceSmallSlice = 8;   // elements: at most this, lanes run over ranks
+
 struct ReduceFunctor {
     const CommImpl* c;
     int64_t lo, hi;
@@ -81,6 +94,41 @@ struct ReduceFunctor {
     __device__ void run() {
         const int P = c->size;
         const CollSlot* slots = c->slots;
+        if constexpr (sizeof(T) <= 8) {
+            if (hi - lo <= kReduceSmallSlice) {
+                // A few elements over many ranks: lanes load the inputs of 64 ranks each
+                // (8 x 64 loads in flight), and the fold runs in rank order 0..P-1 on values
+                // read out of the lanes -- the same order and operations as below.
+                constexpr int B = 8;
+                for (int64_t i = lo; i < hi; i++) {
+                    T acc{};
+                    for (int r0 = 0; r0 < P; r0 += B * kWave) {
+                        T x[B];
+#pragma unroll
+                        for (int b = 0; b < B; b++) {
+                            const int r = r0 + b * kWave + lane();
+                            x[b] = r < P ? ((const T*)slots[r].sbuf)[i] : T{};
+                        }
+#pragma unroll
+                        for (int b = 0; b < B; b++) {
+                            for (int j = 0; j < kWave; j++) {
+                                const int r = r0 + b * kWave + j;
+                                if (r < P) {
+                                    const T v = lane_value(x[b], j);
+                                    acc = r == 0 ? v : combine<T, OP>(v, acc);
+                                }
+                            }
+                        }
+                    }
+                    if (root >= 0) {
+                        if (lane() == 0) ((T*)slots[root].rbuf)[i] = acc;
+                    } else {
+                        for (int d = lane(); d < P; d += kWave) ((T*)slots[d].rbuf)[i] = acc;
+                    }
+                }
+                return;
+            }
+        }
         for (int64_t i = lo + lane(); i < hi; i += kWave) {
             T acc = ((const T*)slots[0].sbuf)[i];
             int r = 1;
@@ -130,10 +178,12 @@ __device__ int coll_reduce(const void* sendbuf, void* recvbuf, int64_t count, in
     if (root < 0 || me == root) out = st.out(recvbuf, bytes);
     uint64_t seq = coll_begin(c, me);
     coll_publish(c, me, seq, in, out, nullptr, nullptr, 0, 0, 0, 0);
-    if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
     int64_t lo, hi;
     reduce_slice(count, P, me, &lo, &hi);
     if (lo < hi) {
+        // only ranks that reduce a slice read the others' buffers; the rest go straight
+        // to the closing barrier, which keeps every buffer alive until the reads are done
+        if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
         if (op->builtin >= 0) {
             ReduceFunctor f{c, lo, hi, root};
             dispatch_op(dt, op->builtin, f);
