@@ -10,6 +10,7 @@
 #include "programs/workloads.h"
 #include "programs/bucket_sort.h"
 #include "programs/serve.h"
+#include "programs/runtime_tests.h"
 #include "../../include/gpumpi/gpumpi.h"
 
 #include <algorithm>
@@ -153,6 +154,8 @@ static const ProgramEntry kPrograms[] = {
     GPUMPI_PROGRAM("t_comm_rank", progs::t_comm_rank),
     GPUMPI_PROGRAM("t_comm_dup", progs::t_comm_dup),
     GPUMPI_PROGRAM("t_comm_split", progs::t_comm_split),
+    GPUMPI_PROGRAM("t_op_table", progs::t_op_table),
+    GPUMPI_PROGRAM("t_lds_buffers", progs::t_lds_buffers),
     GPUMPI_PROGRAM("mcpi", progs::w_mcpi),
     GPUMPI_PROGRAM("p2p", progs::w_p2p),
     GPUMPI_PROGRAM("coll", progs::w_coll),
@@ -436,6 +439,40 @@ static int stage_args(gpumpi_runtime* rt, const void* args, size_t bytes, hipStr
     return GPUMPI_OK;
 }
 
+// Launches a rank grid of `nranks` wavefronts (ranks_per_block per workgroup).  All ranks
+// must be resident at once, since they spin-wait on each other; the reference checks its
+// grid against the occupancy limit (gpu_main/main.cu:189-209).  Here a grid may use at
+// most half of the wave slots its kernel can occupy on the chip.  A dispatch that needs
+// every slot is not guaranteed to be placed all at once: workgroups are spread evenly
+// over the XCDs and their shader arrays while harvested CUs make the arrays' capacities
+// unequal, so an array can run out of room while others still have it (probes at exactly
+// 32 waves per CU placed the last workgroups only after the first ones had given up at
+// their barrier), and a rank that is never placed leaves the rest spinning until the
+// bounded waits report it.  Rank programs use at most 256 registers per lane where the
+// runtime alone is concerned (two waves per SIMD: 1024 ranks on 256 CUs); a program whose
+// own loops need more holds one wave per SIMD and runs up to 512 ranks.
+static int launch_rank_grid(gpumpi_runtime* rt, const void* kernel, int nranks, int argc,
+                            char** d_argv, hipStream_t s) {
+    const int rpb = rt->cfg.ranks_per_block;
+    const int block = 64 * rpb;
+    const int grid = (nranks + rpb - 1) / rpb;
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+    const int64_t slots = (int64_t)per_cu * rpb * rt->cus;  // wave slots this kernel can use
+    const int64_t need = (int64_t)grid * rpb;
+    if (need > slots / 2) {
+        set_err(rt, "rank grid of " + std::to_string(need) + " wavefronts exceeds half of the " +
+                        std::to_string(slots) + " wave slots this program can occupy (" +
+                        std::to_string(per_cu) + " workgroups of " + std::to_string(rpb) +
+                        " per CU)");
+        return GPUMPI_E_RESIDENCY;
+    }
+    Runtime* d = rt->d_rt;
+    void* kargs[] = {(void*)&d, (void*)&argc, (void*)&d_argv};
+    HIP_TRY(hipLaunchKernel(kernel, dim3(grid), dim3(block), kargs, 0, s));
+    return GPUMPI_OK;
+}
+
 static int launch_program(gpumpi_runtime* rt, const ProgramEntry* pe, int argc, char** d_argv,
                           hipStream_t s) {
     if (rt->world_size != rt->cfg.num_ranks) {
@@ -444,22 +481,7 @@ static int launch_program(gpumpi_runtime* rt, const ProgramEntry* pe, int argc, 
                            rt->cfg.num_ranks);
         rt->world_size = rt->cfg.num_ranks;
     }
-    const int rpb = rt->cfg.ranks_per_block;
-    const int block = 64 * rpb;
-    const int grid = (rt->cfg.num_ranks + rpb - 1) / rpb;
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pe->kernel, block, 0));
-    // All ranks must be resident at once (spin-waits on each other), as the reference
-    // requires of its grid (gpu_main/main.cu:189-209).
-    if ((int64_t)per_cu * rt->cus < grid) {
-        set_err(rt, "rank grid of " + std::to_string(grid) + " workgroups exceeds the " +
-                        std::to_string((int64_t)per_cu * rt->cus) + " that can be co-resident");
-        return GPUMPI_E_RESIDENCY;
-    }
-    Runtime* d = rt->d_rt;
-    void* kargs[] = {(void*)&d, (void*)&argc, (void*)&d_argv};
-    HIP_TRY(hipLaunchKernel(pe->kernel, dim3(grid), dim3(block), kargs, 0, s));
-    return GPUMPI_OK;
+    return launch_rank_grid(rt, pe->kernel, rt->cfg.num_ranks, argc, d_argv, s);
 }
 
 int gpumpi_run_async(gpumpi_runtime* rt, const char* program, const void* args,
@@ -1428,7 +1450,8 @@ int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t
     bool ok = halloc((void**)&h.q, sizeof(progs::JobDesc) * queue_capacity) &&
               halloc((void**)&h.q_tail, 64) &&
               halloc((void**)&h.done, sizeof(progs::JobResult) * queue_capacity) &&
-              halloc((void**)&h.done_count, 64) && halloc((void**)&h.stop, 64) &&
+              halloc((void**)&h.done_count, 64) && halloc((void**)&h.admit_head, 64) &&
+              halloc((void**)&h.stop, 64) &&
               halloc((void**)&h.heartbeat, 64) &&
               dalloc((void**)&h.dq, sizeof(progs::JobDesc) * queue_capacity) &&
               dalloc((void**)&h.boxes, sizeof(progs::SlotBox) * S) &&
@@ -1499,10 +1522,18 @@ int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t
     return GPUMPI_OK;
 }
 
+// Two rings of qcap entries carry a job: the submission ring (host ring q, copied into the
+// device's dq) and the completion ring done.  Job i may take position i % qcap only when
+// job i - qcap no longer needs either: the scheduler has admitted or rejected it (its
+// admission head is past it, so dq no longer holds a waiting job there), and this host
+// has polled its result (so done[] holds no unread result there).  The head, not the
+// completion count, bounds the first ring, since backfilled jobs can complete while the
+// job at the head still waits.
 int gpumpi_serve_submit(gpumpi_server* sv, const gpumpi_job* job) {
     if (!sv || !job || !sv->running) return GPUMPI_E_ARG;
-    const uint64_t done = __atomic_load_n(sv->sh.done_count, __ATOMIC_ACQUIRE);
-    if (sv->submitted - done >= (uint64_t)sv->qcap) return GPUMPI_E_NOMEM;  // ring full
+    const uint64_t head = __atomic_load_n(sv->sh.admit_head, __ATOMIC_ACQUIRE);
+    if (sv->submitted - head >= (uint64_t)sv->qcap || sv->submitted - sv->polled >= (uint64_t)sv->qcap)
+        return GPUMPI_E_NOMEM;  // ring full: poll, or wait for admissions
     progs::JobDesc d{job->id, job->seed, job->type, job->ranks, job->size};
     sv->sh.q[sv->submitted % (uint64_t)sv->qcap] = d;
     sv->submitted++;
@@ -1576,14 +1607,7 @@ int gpumpi_run_job(gpumpi_runtime* rt, const gpumpi_job* job, gpumpi_job_result*
     int rc = stage_args(rt, &a, sizeof(a), s);
     if (rc != GPUMPI_OK) return rc;
     const ProgramEntry* pe = find_program("job");
-    const int rpb = rt->cfg.ranks_per_block;
-    const int grid = (job->ranks + rpb - 1) / rpb;
-    Runtime* d = rt->d_rt;
-    int argc = 0;
-    char** argv = nullptr;
-    void* kargs[] = {(void*)&d, (void*)&argc, (void*)&argv};
-    HIP_TRY(hipLaunchKernel(pe->kernel, dim3(grid), dim3(64 * rpb), kargs, 0, s));
-    return GPUMPI_OK;
+    return launch_rank_grid(rt, pe->kernel, job->ranks, 0, nullptr, s);
 }
 
 int64_t gpumpi_job_heap_bytes(const gpumpi_job* job) {

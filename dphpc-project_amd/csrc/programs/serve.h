@@ -235,6 +235,8 @@ struct ServeShared {
     uint64_t* q_tail;            // number of jobs submitted (host release)
     JobResult* done;             // [qcap] completion ring written by the device
     uint64_t* done_count;        // number of completions (device release)
+    uint64_t* admit_head;        // first job the scheduler has not admitted or rejected yet:
+                                 // dq[] entries from here on are still needed (device store)
     int32_t* stop;               // host sets 1: finish admitted jobs and exit
     uint64_t* heartbeat;         // bumped by every host submit/poll call
     // device memory
@@ -309,6 +311,7 @@ __device__ void serve_scheduler(const ServeShared& S) {
     for (int k = lane(); k < S.jcap; k += kWave) S.free_jobs[k] = S.jcap - 1 - k;
     int nfree_jobs = S.jcap;
     uint64_t head = 0;          // first job not yet admitted or rejected
+    uint64_t head_published = 0;
     uint64_t fetched = 0;       // jobs copied from the host ring into dq
     uint64_t completed = 0;
     int running = 0;
@@ -480,6 +483,14 @@ __device__ void serve_scheduler(const ServeShared& S) {
             if (qi == head) head++;
         }
         while (head < tail && S.dq[head % (uint64_t)S.qcap].id == ~0ull) head++;
+        if (head != head_published) {
+            // the host may reuse ring positions below head; the fence orders this wave's
+            // reads of dq[] before the store
+            fence_release();
+            if (lane() == 0)
+                __hip_atomic_store(S.admit_head, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            head_published = head;
+        }
         if (stop && head >= tail && running == 0) break;
         __builtin_amdgcn_s_sleep(2);
     }

@@ -230,7 +230,7 @@ __device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int
     if (uniform) {
         sext = (uint64_t)uniform_scount * se * P;
         rext = (uint64_t)uniform_rcount * re * P;
-    } else if (is_private(sendbuf) || is_private(recvbuf)) {
+    } else if (needs_staging(sendbuf) || needs_staging(recvbuf)) {
         int64_t sm = 0, rm = 0;
         for (int i = lane(); i < P; i += kWave) {
             int64_t a = (int64_t)sdispls[i] + scounts[i];

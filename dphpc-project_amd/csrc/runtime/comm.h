@@ -113,9 +113,12 @@ __device__ bool coll_wait_all(CommImpl* c, uint64_t seq) {
     return true;
 }
 
-// Staging of private-memory (stack) buffers for the pull-based collectives: other ranks
-// cannot address this wavefront's scratch memory, so inputs are copied to the rank's
-// HBM scratch area (or the heap) and outputs are produced there and copied back.
+// Staging of private-memory (stack) and LDS buffers for the pull-based collectives: other
+// ranks cannot address this wavefront's scratch memory or this workgroup's LDS, so inputs
+// are copied to the rank's HBM scratch area (or the heap) and outputs are produced there
+// and copied back.
+__device__ __forceinline__ bool needs_staging(const void* p) { return is_private(p) || is_shared(p); }
+
 struct Stager {
     char* base;
     uint64_t used;
@@ -137,7 +140,7 @@ struct Stager {
         return p;
     }
     __device__ const void* in(const void* p, uint64_t bytes) {
-        if (!p || bytes == 0 || !is_private(p)) return p;
+        if (!p || bytes == 0 || !needs_staging(p)) return p;
         void* a = area(bytes);
         if (!a) {
             GPUMPI_FAIL(kErrResource, (int32_t)bytes, "no staging space for a private buffer");
@@ -147,7 +150,7 @@ struct Stager {
         return a;
     }
     __device__ void* out(void* p, uint64_t bytes) {
-        if (!p || bytes == 0 || !is_private(p)) return p;
+        if (!p || bytes == 0 || !needs_staging(p)) return p;
         void* a = area(bytes);
         if (!a) GPUMPI_FAIL(kErrResource, (int32_t)bytes, "no staging space for a private buffer");
         return a ? a : p;

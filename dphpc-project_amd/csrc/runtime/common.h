@@ -122,10 +122,18 @@ __device__ __forceinline__ bool cas(T* p, T& expected, T desired) {
 // Orders this wave's payload stores before a following flag store (all lanes call it).
 __device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
-// Waits until this wave's loads (and stores) have completed, without writing the L2 back:
-// enough before a store that only says "your data has been read" (the reads returned, so
-// no later store of another wave can change them) and publishes nothing of its own.
-__device__ __forceinline__ void wait_reads_done() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+// Waits until every load and store this wave has issued has completed, without writing
+// the L2 back.  The workgroup-scope fence keeps the compiler from moving memory operations
+// across this point but emits no wait on gfx950 (all waves of a workgroup share one CU),
+// so the counter wait is explicit: s_waitcnt with vmcnt, expcnt and lgkmcnt all zero.
+// Used before a flag store that either only says "your data has been read" (the reads
+// returned, so no later store of another wave can change what they saw) or publishes data
+// written with agent-scope atomic stores (sc1: written through to memory, so once they
+// have completed they are visible to every XCD and need no L2 write-back).
+__device__ __forceinline__ void wait_mem_done() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+}
 
 // ---------------------------------------------------------------- errors
 enum ErrorCode : int32_t {

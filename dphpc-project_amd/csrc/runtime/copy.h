@@ -88,9 +88,10 @@ __device__ void wave_copy_global(char* d, const char* s, uint64_t n, bool nt) {
 }
 
 // Byte-strided copy where lane l moves bytes l, l+64, ...  Used when the source is
-// this wave's private memory: every lane reads its own private copy, and wave-uniform
-// execution keeps all copies identical.
-__device__ __forceinline__ void wave_copy_from_private(char* d, const char* s, uint64_t n) {
+// this wave's private memory (every lane reads its own private copy, and wave-uniform
+// execution keeps all copies identical) or when either side is LDS: generic pointers
+// compile to flat loads and stores, which reach LDS as well as global memory.
+__device__ __forceinline__ void wave_copy_bytes(char* d, const char* s, uint64_t n) {
     for (uint64_t i = (uint64_t)lane(); i < n; i += kWave) d[i] = s[i];
 }
 
@@ -103,19 +104,38 @@ __device__ __forceinline__ void lane_copy_to_private(char* d, const char* s, uin
     }
 }
 
-// General copy between any generic pointers.  LDS buffers are not supported (an LDS
-// address is only meaningful inside one workgroup).
+// Copies of more than kWave * 2 bytes.  Out of line on purpose: inlined into the progress
+// engine and every collective, the unrolled 16-byte loops raised each rank program's
+// register count past 256 per lane (one wave per SIMD, so a 1024-rank grid needed every
+// wave slot of the chip); called, they cost one branch per large message.
+__device__ __noinline__ void wave_copy_large(char* d, const char* s, uint64_t n) {
+    if (is_private(d)) {
+        lane_copy_to_private(d, s, n);
+    } else if (is_private(s) || is_shared(s) || is_shared(d)) {
+        wave_copy_bytes(d, s, n);
+    } else {
+        wave_copy_global<4>(d, s, n, n >= kNontemporalBytes);
+    }
+}
+
+// General copy between any generic pointers (global, this wave's private memory, or LDS
+// of this workgroup).  All 64 lanes must call with identical arguments.  Up to 128 bytes
+// (inline envelopes, small collective slices) every lane moves at most two bytes, both
+// loads issued before the stores.
 __device__ __forceinline__ void wave_copy(void* dst, const void* src, uint64_t n) {
     if (n == 0 || dst == src) return;
     char* d = (char*)dst;
     const char* s = (const char*)src;
-    if (is_private(d)) {
-        lane_copy_to_private(d, s, n);
-    } else if (is_private(s)) {
-        wave_copy_from_private(d, s, n);
-    } else {
-        wave_copy_global<4>(d, s, n, n >= kNontemporalBytes);
+    if (n <= 2 * (uint64_t)kWave && !is_private(d)) {
+        const uint64_t l = (uint64_t)lane();
+        char a = 0, b = 0;
+        if (l < n) a = s[l];
+        if (l + kWave < n) b = s[l + kWave];
+        if (l < n) d[l] = a;
+        if (l + kWave < n) d[l + kWave] = b;
+        return;
     }
+    wave_copy_large(d, s, n);
 }
 
 }  // namespace gpumpi

@@ -221,7 +221,9 @@ __device__ __forceinline__ void heap_account(Heap* h, HeapCache* cache, int64_t 
     }
 }
 
-__device__ void* heap_alloc(Heap* h, HeapCache* cache, uint64_t n) {
+// Out of line (as heap_free): inlined into the progress engine and the collectives, the
+// stack and cache paths added ~70 registers per lane to every rank program.
+__device__ __noinline__ void* heap_alloc(Heap* h, HeapCache* cache, uint64_t n) {
     if (n == 0) return nullptr;
     const int c = heap_class_of(n);
     if (c >= kHeapClasses) return nullptr;
@@ -285,7 +287,7 @@ __device__ uint64_t heap_block_bytes(Heap* h, const void* p) {
     return heap_class_bytes(c);
 }
 
-__device__ void heap_free(Heap* h, HeapCache* cache, void* p) {
+__device__ __noinline__ void heap_free(Heap* h, HeapCache* cache, void* p) {
     if (!p) return;
     const char* cp = (const char*)p;
     if (cp < h->base || cp >= h->base + h->bytes) {

@@ -37,7 +37,9 @@ __device__ __forceinline__ int world_rank() {
 __device__ __forceinline__ RankState* my_state() { return &rt()->ranks[world_rank()]; }
 __device__ __forceinline__ bool aborted() { return ld_relaxed(&rt()->abort_flag) != 0; }
 
-__device__ void record_error(int32_t code, int line, int32_t detail, const char* msg) {
+// Out of line: inlined with a constant message, the character loop unrolls into every
+// caller and the progress engine's register count doubled.
+__device__ __noinline__ void record_error(int32_t code, int line, int32_t detail, const char* msg) {
     Runtime* R = rt();
     if (lane() == 0) {
         int32_t expected = 0;
@@ -256,7 +258,7 @@ __device__ int progress() {
         }
         // the slot goes back to its producers once our reads of it are done; what we
         // wrote (the receive buffer, the unexpected queue) is ours and needs no write-back
-        wait_reads_done();
+        wait_mem_done();
         if (lane() == 0) st_relaxed(&e->seq, (uint32_t)(h + N));
         h++;
         rs->ring_head = h;
@@ -287,7 +289,7 @@ __device__ bool push_envelope(int dest, u32x4 v, bool self_contained) {
         if (!w.keep_going(__LINE__)) return false;
     }
     // no acquire fence: the consumer's reads of this slot completed before it handed the
-    // slot back (wait_reads_done in progress), and we only write the slot
+    // slot back (wait_mem_done in progress), and we only write the slot
     // lane 0 must not overwrite seq with the plain store; it writes words 1..3 only
     if (self_contained) {
         if (lane() >= 1 && lane() < 8) {
@@ -299,7 +301,7 @@ __device__ bool push_envelope(int dest, u32x4 v, bool self_contained) {
             st_relaxed((uint32_t*)e + 1, v.y);
             st_relaxed((uint64_t*)e + 1, ((uint64_t)v.w << 32) | v.z);
         }
-        wait_reads_done();  // the stores above have completed
+        wait_mem_done();  // the sc1 stores above have reached memory before seq is written
     } else {
         if (lane() >= 1 && lane() < 8) ((u32x4*)e)[lane()] = v;
         if (lane() == 0) {

@@ -61,6 +61,22 @@ class BucketSortArgs(ctypes.Structure):
                 ("nlocal", _P), ("passed", _P), ("total", _P), ("seconds", _P)]
 
 
+class OpCase(ctypes.Structure):
+    """One (datatype, operator) case of t_op_table: op is the built-in operator index
+    (MPI_MAX = 0 .. MPI_MINLOC = 11), -1 the int64 user operator, -2 the double one."""
+    _fields_ = [("dtype", ctypes.c_int32), ("op", ctypes.c_int32)]
+
+
+class OpTableArgs(ctypes.Structure):
+    _fields_ = [("cases", _P), ("ncases", ctypes.c_int32), ("count", ctypes.c_int32),
+                ("root", ctypes.c_int32), ("stride", ctypes.c_int32), ("inp", _P),
+                ("out_all", _P), ("out_root", _P), ("status", _P)]
+
+
+class LdsArgs(ctypes.Structure):
+    _fields_ = [("ok", _P)]
+
+
 COLL_KINDS = {"bcast": 0, "reduce": 1, "allreduce": 2, "alltoall": 3, "allgather": 4,
               "gather": 5, "scatter": 6}
 

@@ -269,7 +269,9 @@ typedef struct gpumpi_server gpumpi_server;
 
 int gpumpi_serve_start(const gpumpi_config* cfg, int32_t queue_capacity, int32_t max_job_ranks,
                        double heap_fraction, gpumpi_server** out);
-/* Non-blocking; GPUMPI_E_NOMEM when queue_capacity jobs are already outstanding. */
+/* Non-blocking; GPUMPI_E_NOMEM while the ring is full: queue_capacity jobs submitted since
+ * the oldest job the server has not yet admitted, or since the oldest result this host
+ * has not yet polled.  Poll, then submit again. */
 int gpumpi_serve_submit(gpumpi_server* server, const gpumpi_job* job);
 /* Copies up to max new completions; returns how many. */
 int gpumpi_serve_poll(gpumpi_server* server, gpumpi_job_result* out, int32_t max);

@@ -100,18 +100,40 @@ void orc_fill_int(uint64_t seed, int rank, uint32_t iter, int32_t *out, int64_t 
 
 /* Approximate N(0,1) as Irwin-Hall(12) - 6 evaluated in exact integer
  * arithmetic, so that every platform draws bit-identical doubles. */
-void orc_fill_normal(uint64_t seed, int rank, uint32_t iter, double *out, int64_t n)
+static void orc_fill_normal_one(uint64_t seed, int rank, uint32_t iter, int64_t i, double *out)
 {
     uint32_t o[4];
-    for (int64_t i = 0; i < n; i++) {
-        uint64_t s = 0;
-        for (int j = 0; j < 3; j++) {
-            philox_at(seed, rank, iter, (uint64_t)(3 * i + j), ORC_STREAM_NORMAL, o);
-            s += (uint64_t)o[0] + o[1] + o[2] + o[3];
-        }
-        int64_t c = (int64_t)s - (int64_t)6 * 4294967296LL;
-        out[i] = (double)c * (1.0 / 4294967296.0);
+    uint64_t s = 0;
+    for (int j = 0; j < 3; j++) {
+        philox_at(seed, rank, iter, (uint64_t)(3 * i + j), ORC_STREAM_NORMAL, o);
+        s += (uint64_t)o[0] + o[1] + o[2] + o[3];
     }
+    int64_t c = (int64_t)s - (int64_t)6 * 4294967296LL;
+    *out = (double)c * (1.0 / 4294967296.0);
+}
+
+void orc_fill_normal(uint64_t seed, int rank, uint32_t iter, double *out, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) orc_fill_normal_one(seed, rank, iter, i, &out[i]);
+}
+
+/* Elements idx[0..n) of the orc_fill_int / orc_fill_normal streams: element i depends
+ * only on its own Philox counter(s), so samples of a long input are cheap to draw. */
+void orc_fill_int_at(uint64_t seed, int rank, uint32_t iter, const int64_t *idx, int64_t n,
+                     int32_t *out)
+{
+    uint32_t o[4];
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t i = idx[k];
+        philox_at(seed, rank, iter, (uint64_t)(i >> 2), ORC_STREAM_INT, o);
+        out[k] = (int32_t)(o[i & 3] % 2097153u) - 1048576;
+    }
+}
+
+void orc_fill_normal_at(uint64_t seed, int rank, uint32_t iter, const int64_t *idx, int64_t n,
+                        double *out)
+{
+    for (int64_t k = 0; k < n; k++) orc_fill_normal_one(seed, rank, iter, idx[k], &out[k]);
 }
 
 void orc_fill_keys_uniform(uint64_t seed, int rank, uint32_t step, uint32_t *out,

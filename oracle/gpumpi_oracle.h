@@ -79,6 +79,11 @@ int64_t orc_mcpi_hits(uint64_t seed, int rank, int64_t n);
 void orc_fill_bytes(uint64_t seed, int rank, uint32_t iter, uint8_t *buf, int64_t nbytes);
 void orc_fill_int(uint64_t seed, int rank, uint32_t iter, int32_t *out, int64_t n);
 void orc_fill_normal(uint64_t seed, int rank, uint32_t iter, double *out, int64_t n);
+/* The same streams at the element indices idx[0..n). */
+void orc_fill_int_at(uint64_t seed, int rank, uint32_t iter, const int64_t *idx, int64_t n,
+                     int32_t *out);
+void orc_fill_normal_at(uint64_t seed, int rank, uint32_t iter, const int64_t *idx, int64_t n,
+                        double *out);
 void orc_fill_keys_uniform(uint64_t seed, int rank, uint32_t step, uint32_t *out,
                            int64_t n, int key_bits);
 void orc_fill_keys_zipf(uint64_t seed, int rank, uint32_t step, const double *cdf,

@@ -7,8 +7,8 @@
 #include <gpumpi/mpi.h>
 #include <gpumpi/gpu_malloc.h>
 
-__device__ int __gpu_max_threads() { return 1024; }
-__device__ int visits[1024];
+__device__ int __gpu_max_threads() { return 512; }
+__device__ int visits[512];
 
 __device__ double f(double a) { return 4.0 / (1.0 + a * a); }
 

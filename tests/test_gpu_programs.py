@@ -28,11 +28,19 @@ def test_converted_pi(ranks):
 
 
 def test_converted_program_rejects_too_many_ranks():
-    """More ranks than the per-rank global arrays hold: the launch fails cleanly
-    (main.cu:113-122 behaviour) instead of corrupting memory."""
-    r = run(["5", "---gpumpi", "-n", "1100", "-p", str(64 << 20), "-t", "5"])
+    """More ranks than the per-rank global arrays hold (the program is compiled for 512):
+    the launch fails cleanly (main.cu:113-122 behaviour) instead of corrupting memory."""
+    r = run(["5", "---gpumpi", "-n", "600", "-p", str(64 << 20), "-t", "5"])
     assert r.returncode != 0
     assert "GPU_MPI_MAX_RANKS" in (r.stdout + r.stderr) or "device error" in (r.stdout + r.stderr)
+
+
+def test_rank_grid_beyond_half_the_wave_slots_is_refused():
+    """A grid that needs more than half of the wave slots its kernel can occupy is refused
+    before launch (the reference's co-residency check, main.cu:189-209, with headroom)."""
+    r = run(["5", "---gpumpi", "-n", "1100", "-p", str(64 << 20), "-t", "5"])
+    assert r.returncode != 0
+    assert "exceeds half of the" in r.stderr, r.stderr
 
 
 def test_cli_requires_gpumpi_separator():
