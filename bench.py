@@ -5,18 +5,22 @@ Workload (the bucket-sort exchange): P = 512 ranks, 2^18 uint32 keys per rank (2
 keys, 512 MiB) uniform in [0, 2^23), bucketed NPB-style into 1024 buckets (NPB IS
 rank(): histogram, splitters, stable bucketing).  One timed step = the batched exchange:
 Alltoall of the counts + receive displacements + the skewed Alltoallv of the keys, as
-three kernels (plan, scan, copy), replayed from a hipGraph that holds 10 steps (one graph
-launch per 10 steps).  Inputs are resident in HBM before the timed region.
+plan, scan and copy work replayed from hipGraphs of 10 steps (exactly --steps steps are
+timed: replays of the 10-step graph and one graph of the remainder).  Inside a graph each
+step's copy kernel also plans the next step into a second workspace, so the plan and the
+chunk scan leave the critical path.  Inputs are resident in HBM before the timed region.
 
 value  = algorithmic bytes / time = 2 x payload bytes (read once + written once) per
          step / seconds per step, in GB/s.
-roofline = the copy kernel alone (HIP events around it on its stream, averaged): the
-         same algorithmic bytes / its duration, against the 8 TB/s HBM peak.
+roofline = the dominant kernel of the timed steps alone -- the copy kernel that also plans
+         the next step (HIP events around it on its stream, averaged): 2 x payload bytes /
+         its duration, against the 8 TB/s HBM peak.
 Also reported: the plain per-call path (one rank-grid kernel launch per MPI call:
 MPI_Alltoall then MPI_Alltoallv on 512 wavefront ranks) and the ratio, the Zipf(1.1)
 variant, the particle variant of the exchange (2^24 particles on an 8x8x8 rank grid, as a
 rank-grid MPI program and as the batched particle step), the host-MPI CPU baseline (the
-whole 2^27-key exchange under MPICH, one rank per host core, ~10 s of exchanges), and
+whole 2^27-key exchange under MPICH, one rank per core of the CPU share this GPU has,
+~10 s of exchanges), and
 served MPI jobs: Poisson arrivals of independent dot /
 Alltoall jobs of 8..256 ranks at fractions of the plain path's capacity, p50/p99
 latency (arrival -> result on the host) and jobs/s for the device-resident server
@@ -138,6 +142,27 @@ def time_steps(torch, fn, steps):
     return e0.elapsed_time(e1) / steps  # ms per step
 
 
+GRAPH_STEPS = 10
+
+
+def run_steps(plan, steps, stream):
+    """Exactly `steps` steps: replays of the 10-step graph, then one graph of the rest."""
+    full, rem = divmod(steps, GRAPH_STEPS)
+    for _ in range(full):
+        plan.graph_replay(GRAPH_STEPS, stream)
+    if rem:
+        plan.graph_replay(rem, stream)
+
+
+def timed_steps(torch, plan, steps, warmup, stream):
+    """Warm-up (at least one replay of every graph the timed steps use, so no capture
+    happens inside the timed region), then ms per step over exactly `steps` steps."""
+    run_steps(plan, max(warmup, GRAPH_STEPS), stream)
+    if steps % GRAPH_STEPS:
+        plan.graph_replay(steps % GRAPH_STEPS, stream)
+    return time_steps(torch, lambda: run_steps(plan, steps, stream), 1) / steps
+
+
 def kernel_times(torch, plan, stream, reps):
     """Average per-kernel durations of a step (HIP events recorded by the library around
     the plan, scan and copy kernels on the step's stream)."""
@@ -152,15 +177,47 @@ def kernel_times(torch, plan, stream, reps):
     return {"plan_us": acc[0] / reps, "scan_us": acc[1] / reps, "copy_us": acc[2] / reps}
 
 
+def host_info(mpirun):
+    """What the CPU baseline ran on: nproc, the CPU model and the MPI library's version,
+    read on this machine."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    version = "unknown"
+    d = os.path.dirname(mpirun)
+    for cmd in ([os.path.join(d, "mpichversion")], [mpirun, "--version"]):
+        if not os.path.exists(cmd[0]):
+            continue
+        try:
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=30).stdout
+        except (OSError, subprocess.TimeoutExpired):
+            continue
+        for line in out.splitlines():
+            if "Version" in line or "version" in line:
+                version = " ".join(line.split())
+                break
+        if version != "unknown":
+            break
+    return os.cpu_count() or 1, model, version
+
+
 def cpu_baseline(args):
-    """The same exchange under host MPI (MPICH), one rank per host core, timed by the
-    oracle's MPI program.  Returns the cpu_baseline object or None."""
+    """The same exchange under host MPI (MPICH), one rank per core of the CPU share this
+    GPU has (OMP_NUM_THREADS, 16 where unset), timed by the oracle's MPI program.
+    Returns the cpu_baseline object or None."""
     exe = os.path.join(ROOT, "oracle", "_build", "a2av_bucket")
     mpirun = shutil.which("mpirun") or ("/opt/conda/bin/mpirun" if os.path.exists("/opt/conda/bin/mpirun") else None)
     if not os.path.exists(exe) or not mpirun:
         log("cpu baseline skipped: host MPI program or mpirun missing")
         return None
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 64)
+    nproc, model, mpi_version = host_info(mpirun)
+    share = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    cores = min(nproc, share)
     total = args.ranks << args.keys_log2          # the whole workload's keys
     n = total // cores
     cmd = [mpirun, "-n", str(cores), exe, str(n), "0", "0x5eed", str(args.cpu_iters), "2"]
@@ -182,10 +239,13 @@ def cpu_baseline(args):
         f"(wall {time.time()-t0:.1f} s)")
     return {"value": round(2 * payload / secs / 1e9, 3), "unit": "GB/s", "cores": cores,
             "kind": "port",
-            "sample": f"MPICH 3.3.2 MPI_Alltoall+MPI_Alltoallv bucket exchange of the whole workload "
-                      f"({total} uniform uint32 keys) over {cores} ranks (one per host core, {n} "
-                      f"keys each), {args.cpu_iters} timed repetitions ({args.cpu_iters * secs:.1f} s "
-                      f"of exchanges), sorted_ok={kv['sorted_ok']}"}
+            "sample": f"{mpi_version}: MPI_Alltoall+MPI_Alltoallv bucket exchange of the whole "
+                      f"workload ({total} uniform uint32 keys) over {cores} ranks, one per core "
+                      f"(nproc {nproc}; {cores} = min(nproc, OMP_NUM_THREADS={share}), the CPU "
+                      f"share of this GPU), {n} keys each, {args.cpu_iters} timed repetitions "
+                      f"({args.cpu_iters * secs:.1f} s of exchanges), sorted_ok={kv['sorted_ok']}; "
+                      f"CPU: {model}",
+            "host": {"nproc": nproc, "cpu_model": model, "mpi": mpi_version, "ranks": cores}}
 
 
 def make_jobs(pkg, njobs, seed, heap_budget):
@@ -428,6 +488,70 @@ def coll_sweep(pkg, torch, args):
     return out
 
 
+def batched_reduction(pkg, torch, args):
+    """Batched MPI_Allreduce of one step over 1024 ranks (gpumpi_reduce_*): one kernel per
+    step, replayed from graphs of 10 steps, at 1 MiB per rank (MPI_DOUBLE and MPI_INT, SUM)
+    and 8 MiB per rank (MPI_DOUBLE).  Algorithmic bytes 2 * P * N * s per step (every input
+    read once, every rank's result written once).  Against it, the plain per-call path:
+    one rank-grid launch per MPI_Allreduce (the coll program on the same inputs, one call
+    per launch); the two outputs must be the same bytes (both fold in rank order)."""
+    from dphpc_project_amd import programs as pr
+    P = 1024
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"ranks": P}
+    cases = ((13, "MPI_DOUBLE", 8, 1 << 20), (2, "MPI_INT", 4, 1 << 20), (13, "MPI_DOUBLE", 8, 8 << 20))
+    rt = pkg.Runtime(P, ranks_per_block=4, heap_bytes=256 << 20, timeout_s=60.0)
+    try:
+        for dt, dname, es, per_rank in cases:
+            n = per_rank // es
+            inb = torch.zeros((P, per_rank), dtype=torch.uint8, device="cuda")
+            outb = torch.zeros((P, per_rank), dtype=torch.uint8, device="cuda")
+            bout = torch.zeros_like(outb)
+            sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+            a = pr.CollArgs(0xA11, pr.COLL_KINDS["allreduce"], dt, 0, 0, n, 1, 0, inb.data_ptr(),
+                            outb.data_ptr(), per_rank, per_rank, sec.data_ptr(), 0)
+            rt.run("coll", a)  # fills the inputs (Philox) and runs one plain Allreduce
+            torch.cuda.synchronize()
+            a.no_fill = 1
+            plain_ms = time_steps(torch, lambda: rt.run_async("coll", a, stream), 3)
+            sp = torch.tensor([inb[r].data_ptr() for r in range(P)], dtype=torch.int64, device="cuda")
+            rp = torch.tensor([bout[r].data_ptr() for r in range(P)], dtype=torch.int64, device="cuda")
+            plan = pkg.ReducePlan(P, n, dt, 2, -1, sp.data_ptr(), rp.data_ptr())
+            try:
+                steps = 20
+                run = lambda k: [plan.graph_replay(GRAPH_STEPS, stream) for _ in range(k // GRAPH_STEPS)]
+                run(GRAPH_STEPS)
+                ms = time_steps(torch, lambda: run(steps), 1) / steps
+                kus = plan.kernel_time(10, stream)
+            finally:
+                plan.close()
+            torch.cuda.synchronize()
+            same = bool((bout == outb[0][None, :]).all().item()) and bool((outb == outb[0][None, :]).all().item())
+            algo = 2 * P * per_rank
+            gbs = algo / (ms * 1e-3) / 1e9
+            kgbs = algo / (kus * 1e-6) / 1e9
+            key = f"{dname}_{per_rank >> 20}MiB"
+            out[key] = {"count_per_rank": n, "ms_per_step": round(ms, 4), "GBps": round(gbs, 1),
+                        "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                        "kernel_us": round(kus, 2),
+                        "roofline": {"bound": "hbm", "kernel": "batched_reduce_kernel",
+                                     "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(kgbs / HBM_PEAK_GBS, 4)},
+                        "plain_ms_per_call": round(plain_ms, 4),
+                        "plain_GBps": round(algo / (plain_ms * 1e-3) / 1e9, 1),
+                        "batched_vs_plain": round(plain_ms / ms, 2),
+                        "bit_exact_vs_rank_grid": same}
+            log(f"batched Allreduce {dname} {per_rank >> 20} MiB/rank on {P} ranks: {ms * 1e3:.0f} us/step "
+                f"{gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak), kernel {kus:.0f} us; plain "
+                f"{plain_ms * 1e3:.0f} us/call ({plain_ms / ms:.2f}x); identical: {same}")
+            assert same, "batched Allreduce differs from the rank grid's"
+            del inb, outb, bout
+            torch.cuda.empty_cache()
+    finally:
+        rt.close()
+    return out
+
+
 def serve_latency(pkg, torch, args):
     """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
     the host, for one plain rank-grid launch per job and for the device-resident server.
@@ -558,6 +682,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--no-npb", action="store_true")
+    ap.add_argument("--no-reduce", action="store_true")
     ap.add_argument("--no-sweeps", action="store_true")
     ap.add_argument("--churn-ops", type=int, default=1000000)
     ap.add_argument("--serve-jobs", type=int, default=400)
@@ -591,27 +716,35 @@ def main():
     if not args.no_autotune and tune is None:
         chosen, us = plan.autotune(stream)
         log(f"autotune: {chosen} best {us:.1f} us")
-    # steps are replayed from a graph of g captured steps (one graph launch per g steps);
-    # one checked step first, then the warm-up right before the timed steps
-    g = 10 if args.steps % 10 == 0 and args.warmup % 10 == 0 else 1
-    plan.graph_replay(g, stream)
+    # steps are replayed from graphs of 10 captured steps; one checked graph first, then
+    # the warm-up right before the timed steps
+    plan.graph_replay(GRAPH_STEPS, stream)
     torch.cuda.synchronize()
     ex.verify(torch)
     assert plan.payload_bytes() == ex.payload
-    for _ in range(max(1, args.warmup // g)):
-        plan.graph_replay(g, stream)
-    ms = time_steps(torch, lambda: plan.graph_replay(g, stream), args.steps // g) / g
+    ms = timed_steps(torch, plan, args.steps, args.warmup, stream)
     algo = 2 * ex.payload
     gbs = algo / (ms * 1e-3) / 1e9
     log(f"batched: {ms*1e3:.1f} us/step, {gbs:.1f} GB/s ({100*gbs/HBM_PEAK_GBS:.1f}% of HBM peak), "
         f"tune {plan.tune()}")
     ex.verify(torch)
     tune_used = plan.tune()
-    # the dominant kernel (copy) alone: HIP events around it on its own stream, averaged
+    # the kernels of a three-kernel step, and the dominant kernel of the timed steps: the
+    # copy kernel that also plans the next step (HIP events on the launch stream, averaged)
     kern = kernel_times(torch, plan, stream, max(5, args.steps // 5))
-    copy_gbs = algo / (kern["copy_us"] * 1e-6) / 1e9
+    pipe = None
+    if tune_used["pipeline"]:
+        pipe = plan.pipeline_times(max(10, args.steps // 5), stream)
+        dom_us = pipe["copy_us"]
+        dom_name = "a2av_copy_kernel<U, NT, DEFER, FUSE=true> (copy + next step's plan)"
+    else:
+        dom_us = kern["copy_us"]
+        dom_name = "a2av_copy_kernel<U, NT, DEFER, FUSE=false>"
+    copy_gbs = algo / (dom_us * 1e-6) / 1e9
+    ex.verify(torch)
     log(f"kernels: plan {kern['plan_us']:.1f} us, scan {kern['scan_us']:.1f} us, copy "
-        f"{kern['copy_us']:.1f} us = {copy_gbs:.1f} GB/s ({100*copy_gbs/HBM_PEAK_GBS:.1f}% of peak)")
+        f"{kern['copy_us']:.1f} us; pipelined: {pipe}; dominant {dom_us:.1f} us = "
+        f"{copy_gbs:.1f} GB/s ({100*copy_gbs/HBM_PEAK_GBS:.1f}% of peak)")
 
     # ---------------- plain per-call path (one rank-grid launch per MPI call)
     plain = None
@@ -643,16 +776,16 @@ def main():
         ez = Exchange(pkg, torch, P, n, "zipf")
         zp = ez.plan(pkg, (tune_used["block_threads"], tune_used["blocks_per_cu"],
                            tune_used["lines_per_lane"], tune_used["nontemporal"],
-                           tune_used["chunk_rounds"], tune_used["defer_straddle"]))
+                           tune_used["chunk_rounds"], tune_used["defer_straddle"],
+                           tune_used["pipeline"]))
         if not args.no_autotune and tune is None:
             zchosen, zus = zp.autotune(stream)
             log(f"zipf autotune: {zchosen} best {zus:.1f} us")
-        zp.graph_replay(g, stream)
+        zp.graph_replay(GRAPH_STEPS, stream)
         torch.cuda.synchronize()
         ez.verify(torch)
-        for _ in range(max(1, args.warmup // g)):
-            zp.graph_replay(g, stream)
-        zms = time_steps(torch, lambda: zp.graph_replay(g, stream), args.steps // g) / g
+        zms = timed_steps(torch, zp, args.steps, args.warmup, stream)
+        ez.verify(torch)
         zipf = {"ms_per_step": round(zms, 5), "GBps": round(2 * ez.payload / (zms * 1e-3) / 1e9, 2),
                 "max_recv_bytes": int(ez.rtot.max()) * 4, "payload_bytes": ez.payload,
                 "tune": zp.tune()}
@@ -677,6 +810,7 @@ def main():
         log(f"NPB IS class C on 512 ranks: passed_verification {passed} (expected 562), "
             f"{ex_ms / 10:.3f} ms per iteration's exchange")
         assert passed == 5 * 10 + 512, "NPB IS class C failed its verification"
+    reduction = None if args.no_reduce else batched_reduction(pkg, torch, args)
     particles = None if args.no_particles else particle_exchange(pkg, torch, args)
     serve = None if args.no_serve else serve_latency(pkg, torch, args)
     p2p = None if args.no_sweeps else p2p_sweep(pkg, torch, args)
@@ -684,11 +818,16 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(args)
 
-    traffic = None
+    # HBM traffic per launch of the dominant kernel: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # runs of this bench, scaled by a plain copy's counts; read from the committed summary
+    traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", "a2av_copy_kernel_uniform_p512_k262144_traffic.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            tj = json.load(open(tf))
+            if tj.get("kernel_variant", "") == ("fused" if tune_used["pipeline"] else "plain"):
+                traffic = tj.get("hbm_bytes_per_launch")
+                traffic_src = tj.get("source")
         except Exception:
             traffic = None
 
@@ -707,18 +846,22 @@ def main():
         "data": "synthetic (Philox uniform keys, NPB-style buckets)",
         "config": {"workload": f"bucket-sort exchange: {P} ranks x 2^{args.keys_log2} uint32 keys, "
                                "uniform [0,2^23), 1024 buckets; one step = Alltoall(counts) + "
-                               "Alltoallv(keys) as plan/scan/copy kernels, replayed from a hipGraph of 10 steps",
+                               "Alltoallv(keys): plan, scan and copy work, replayed from hipGraphs of "
+                               "10 steps in which each step's copy kernel also plans the next step",
                    "ranks": P, "keys_per_rank": n, "payload_bytes": ex.payload,
-                   "tune": tune_used, "graph_steps": g,
-                   "kernel_us": {k: round(v, 2) for k, v in kern.items()}},
-        "roofline": {"bound": "hbm", "kernel": "a2av_copy_kernel",
+                   "tune": tune_used, "graph_steps": GRAPH_STEPS,
+                   "kernel_us": {k: round(v, 2) for k, v in kern.items()},
+                   "pipelined_kernel_us": {k: round(v, 2) for k, v in pipe.items()} if pipe else None},
+        "roofline": {"bound": "hbm", "kernel": dom_name,
                      "achieved": round(copy_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(copy_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(copy_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "plain_path": plain,
         "batched_vs_plain": round(plain["ms_per_step"] / ms, 2) if plain else None,
         "zipf": zipf,
         "npb_is": npb,
+        "batched_allreduce": reduction,
         "particles": particles,
         "serving": serve,
         "p2p": p2p,

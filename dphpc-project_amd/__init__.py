@@ -62,6 +62,7 @@ class A2avTune(ctypes.Structure):
         ("block_threads", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
         ("lines_per_lane", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
         ("chunk_rounds", ctypes.c_int32), ("defer_straddle", ctypes.c_int32),
+        ("pipeline", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -99,6 +100,13 @@ _SIGNATURES = {
     "gpumpi_a2av_graph_replay": (ctypes.c_int, [_P, _I32, _P]),
     "gpumpi_a2av_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(A2avTune), ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_a2av_pipeline_times": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_reduce_plan_create": (ctypes.c_int, [_I32, ctypes.c_int64, _I32, _I32, _I32, _P, _P,
+                                                 ctypes.POINTER(_P)]),
+    "gpumpi_reduce_plan_destroy": (None, [_P]),
+    "gpumpi_reduce_execute": (ctypes.c_int, [_P, _P]),
+    "gpumpi_reduce_graph_replay": (ctypes.c_int, [_P, _I32, _P]),
+    "gpumpi_reduce_kernel_time": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_get_tune": (ctypes.c_int, [_P, ctypes.POINTER(A2avTune), ctypes.POINTER(_I32)]),
     "gpumpi_a2av_payload_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "gpumpi_particles_plan_create": (ctypes.c_int, [_I32, _I32, _I32, ctypes.c_int64, ctypes.c_uint64,
@@ -348,6 +356,50 @@ def job_heap_bytes(job):
     return load_library().gpumpi_job_heap_bytes(ctypes.byref(job))
 
 
+class ReducePlan:
+    """Batched MPI_Reduce (root >= 0) / MPI_Allreduce (root = -1) of one step over P ranks
+    (gpumpi_reduce_*): sendbufs_ptr / recvbufs_ptr are device arrays of P pointers."""
+
+    def __init__(self, P, count, datatype, op, root, sendbufs_ptr, recvbufs_ptr):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.gpumpi_reduce_plan_create(P, count, datatype, op, root, sendbufs_ptr,
+                                                recvbufs_ptr, ctypes.byref(h))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_reduce_plan_create failed ({rc}): "
+                              f"{self.lib.gpumpi_last_error(None).decode()}")
+        self.h = h
+
+    def execute(self, stream=0):
+        rc = self.lib.gpumpi_reduce_execute(self.h, ctypes.c_void_p(stream) if stream else None)
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_reduce_execute failed ({rc})")
+
+    def graph_replay(self, steps, stream=0):
+        rc = self.lib.gpumpi_reduce_graph_replay(self.h, steps, ctypes.c_void_p(stream) if stream else None)
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_reduce_graph_replay failed ({rc})")
+
+    def kernel_time(self, reps, stream=0):
+        us = ctypes.c_double()
+        rc = self.lib.gpumpi_reduce_kernel_time(self.h, reps, ctypes.c_void_p(stream) if stream else None,
+                                                ctypes.byref(us))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_reduce_kernel_time failed ({rc})")
+        return us.value
+
+    def close(self):
+        if self.h:
+            self.lib.gpumpi_reduce_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class A2avPlan:
     """Batched Alltoallv step: plan, scan and copy kernels, replayed as one hipGraph
     (gpumpi_a2av_*)."""
@@ -355,7 +407,9 @@ class A2avPlan:
     def __init__(self, P, elem_bytes, sendbufs_ptr, scounts_ptr, sdispls_ptr, recvbufs_ptr,
                  rcounts_ptr, rdispls_ptr, tune=None):
         self.lib = load_library()
-        t = A2avTune(0, -1, 0, -1, 0, -1) if tune is None else A2avTune(*tune)
+        t = A2avTune(0, -1, 0, -1, 0, -1, -1) if tune is None else A2avTune(*tune)
+        if tune is not None and len(tune) < 7:
+            t.pipeline = -1
         h = _P()
         rc = self.lib.gpumpi_a2av_plan_create(P, elem_bytes, sendbufs_ptr, scounts_ptr,
                                               sdispls_ptr, recvbufs_ptr, rcounts_ptr,
@@ -402,6 +456,16 @@ class A2avPlan:
         if rc != OK:
             raise GpumpiError("phase times not armed")
         return list(out)
+
+    def pipeline_times(self, steps, stream=0):
+        """Kernel durations of `steps` pipelined steps (HIP events): {"plan_scan_us":
+        first step's plan + scan, "copy_us": average copy kernel that also plans the next
+        step, "step_us": the sequence per step}."""
+        out = (ctypes.c_double * 3)()
+        rc = self.lib.gpumpi_a2av_pipeline_times(self.h, steps, ctypes.c_void_p(stream) if stream else None, out)
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_a2av_pipeline_times failed ({rc})")
+        return {"plan_scan_us": out[0], "copy_us": out[1], "step_us": out[2]}
 
     def payload_bytes(self):
         b = ctypes.c_int64()

@@ -4,6 +4,7 @@
 #include "runtime/mpi_api.h"
 #include "runtime/rank_kernel.h"
 #include "runtime/batched.h"
+#include "runtime/batched_reduce.h"
 #include "runtime/bucket.h"
 #include "runtime/particle_step.h"
 #include "programs/reference_tests.h"
@@ -837,31 +838,38 @@ static int cli_main(const void* kernel, int argc, const char* const* argv) {
 }  // extern "C"
 
 // ==================================================================== batched Alltoallv
+struct A2avGraph {
+    int steps = 0;
+    hipGraph_t def = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct gpumpi_a2av_plan {
-    A2avDev dev{};
+    A2avDev dev{};            // tables of workspace 0 (single steps, the first step of a graph)
+    A2avDev dev2{};           // workspace 1: the other half of the pipelined graph's steps
     gpumpi_a2av_tune tune{};
     int grid = 0;             // copy-kernel workgroups
     int cus = 0;
     int64_t est_bytes = -1;   // payload of the last step read back (sizes the copy grid)
+    int last_ws = 0;          // workspace of the last step launched
     std::vector<void*> allocs;
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
-    int graph_steps = 0;
+    std::vector<A2avGraph> graphs;  // one per step count replayed
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-template <bool DEFER>
+template <bool DEFER, bool FUSE>
 static const void* a2av_copy_for_mode(int U, bool nt) {
     switch (U) {
-    case 2: return nt ? (const void*)&a2av_copy_kernel<2, true, DEFER> : (const void*)&a2av_copy_kernel<2, false, DEFER>;
-    case 4: return nt ? (const void*)&a2av_copy_kernel<4, true, DEFER> : (const void*)&a2av_copy_kernel<4, false, DEFER>;
-    default: return nt ? (const void*)&a2av_copy_kernel<8, true, DEFER> : (const void*)&a2av_copy_kernel<8, false, DEFER>;
+    case 2: return nt ? (const void*)&a2av_copy_kernel<2, true, DEFER, FUSE> : (const void*)&a2av_copy_kernel<2, false, DEFER, FUSE>;
+    case 4: return nt ? (const void*)&a2av_copy_kernel<4, true, DEFER, FUSE> : (const void*)&a2av_copy_kernel<4, false, DEFER, FUSE>;
+    default: return nt ? (const void*)&a2av_copy_kernel<8, true, DEFER, FUSE> : (const void*)&a2av_copy_kernel<8, false, DEFER, FUSE>;
     }
 }
 
-static const void* a2av_copy_for(int U, bool nt, bool defer) {
-    return defer ? a2av_copy_for_mode<true>(U, nt) : a2av_copy_for_mode<false>(U, nt);
+static const void* a2av_copy_for(int U, bool nt, bool defer, bool fuse) {
+    if (fuse) return defer ? a2av_copy_for_mode<true, true>(U, nt) : a2av_copy_for_mode<false, true>(U, nt);
+    return defer ? a2av_copy_for_mode<true, false>(U, nt) : a2av_copy_for_mode<false, false>(U, nt);
 }
 
 // Copy grid: one workgroup per chunk of the last measured step (P extra for the partial
@@ -892,40 +900,80 @@ static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
     pl->tune.nontemporal = t.nontemporal > 0 ? 1 : 0;
     pl->tune.chunk_rounds = R;
     pl->tune.defer_straddle = t.defer_straddle > 0 ? 1 : 0;
+    pl->tune.pipeline = t.pipeline != 0 ? 1 : 0;
     pl->dev.chunk_lines = (uint32_t)(kA2avCopyThreads * U * R);
     a2av_size_grid(pl);
+    pl->dev2.chunk_lines = pl->dev.chunk_lines;
+    pl->dev2.straddle_blocks = pl->dev.straddle_blocks;
     return GPUMPI_OK;
 }
 
+// One step of three kernels on workspace 0: plan, scan, copy.
 static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s, bool timed) {
-    const int P = pl->dev.P;
-    void* kargs[] = {(void*)&pl->dev};
+    A2avDev& d = pl->dev;
+    void* kargs[] = {(void*)&d, (void*)&d};
+    void* kargs1[] = {(void*)&d};
     if (timed) hipEventRecord(pl->ev[0], s);
-    if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(P), dim3(256), kargs, 0, s) != hipSuccess)
+    if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(d.P), dim3(256), kargs1, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
     if (timed) hipEventRecord(pl->ev[1], s);
-    if (hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs, 0, s) != hipSuccess)
+    if (hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
     if (timed) hipEventRecord(pl->ev[2], s);
     if (hipLaunchKernel(a2av_copy_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0,
-                                      pl->tune.defer_straddle > 0),
-                        dim3(pl->grid),
-                        dim3(kA2avCopyThreads), kargs, 0, s) != hipSuccess)
+                                      pl->tune.defer_straddle > 0, false),
+                        dim3(pl->grid), dim3(kA2avCopyThreads), kargs, 0, s) != hipSuccess)
         return GPUMPI_E_HIP;
     if (timed) hipEventRecord(pl->ev[3], s);
+    pl->last_ws = 0;
     return GPUMPI_OK;
 }
 
+// The copy of the step whose tables are in workspace `cur`, with the plan of the next
+// step (into the other workspace) in its first P workgroups when `plan_next`.
+static int a2av_launch_copy(gpumpi_a2av_plan* pl, hipStream_t s, int cur, bool plan_next) {
+    A2avDev& a = cur == 0 ? pl->dev : pl->dev2;
+    A2avDev& b = cur == 0 ? pl->dev2 : pl->dev;
+    void* kargs[] = {(void*)&a, (void*)&b};
+    const int grid = pl->grid + (plan_next ? a.P : 0);
+    if (hipLaunchKernel(a2av_copy_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0,
+                                      pl->tune.defer_straddle > 0, plan_next),
+                        dim3(grid), dim3(kA2avCopyThreads), kargs, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    pl->last_ws = cur;
+    return GPUMPI_OK;
+}
+
+// `steps` steps.  Pipelined: plan and scan of the first step, then one copy kernel per
+// step that also plans the next step into the other workspace (the last one plans
+// nothing).  Otherwise three kernels per step.
+static int a2av_launch_steps(gpumpi_a2av_plan* pl, hipStream_t s, int steps) {
+    int rc = GPUMPI_OK;
+    if (pl->tune.pipeline) {
+        A2avDev& d = pl->dev;
+        void* kargs1[] = {(void*)&d};
+        if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(d.P), dim3(256), kargs1, 0, s) != hipSuccess ||
+            hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess)
+            return GPUMPI_E_HIP;
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++)
+            rc = a2av_launch_copy(pl, s, i & 1, i + 1 < steps);
+    } else {
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) rc = a2av_launch(pl, s, false);
+    }
+    return rc;
+}
+
 static void a2av_drop_graph(gpumpi_a2av_plan* pl) {
-    if (pl->graph) hipGraphExecDestroy(pl->graph);
-    if (pl->graph_def) hipGraphDestroy(pl->graph_def);
-    pl->graph = nullptr;
-    pl->graph_def = nullptr;
-    pl->graph_steps = 0;
+    for (A2avGraph& g : pl->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.def) hipGraphDestroy(g.def);
+    }
+    pl->graphs.clear();
 }
 
 static int a2av_read_bytes(gpumpi_a2av_plan* pl, int64_t* bytes) {
-    if (hipMemcpy(bytes, pl->dev.total_bytes, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
+    const A2avDev& d = pl->last_ws == 0 ? pl->dev : pl->dev2;
+    if (hipMemcpy(bytes, d.total_bytes, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
         return GPUMPI_E_HIP;
     return GPUMPI_OK;
 }
@@ -956,21 +1004,29 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
     d.rcounts = recv_counts;
     d.rdispls = recv_displs;
     const uint64_t PP = (uint64_t)P * (uint64_t)P;
+    // per workspace: seg_end, seg_src [P*P], row_bytes [P], chunk_off [P+1], total bytes,
+    // plan-done counter (on its own 128-byte line)
     const size_t ws = sizeof(uint64_t) * (2 * PP + P + (P + 1)) + 256;
     char* w = nullptr;
-    if (hipMalloc((void**)&w, ws) != hipSuccess) {
+    if (hipMalloc((void**)&w, 2 * ws) != hipSuccess) {
         delete pl;
         return GPUMPI_E_NOMEM;
     }
     pl->allocs.push_back(w);
-    hipMemset(w, 0, ws);
-    d.seg_end = (uint64_t*)w;
-    d.seg_src = d.seg_end + PP;
-    d.row_bytes = d.seg_src + PP;
-    d.chunk_off = d.row_bytes + P;
-    d.total_bytes = (int64_t*)(d.chunk_off + P + 1);
+    hipMemset(w, 0, 2 * ws);
+    pl->dev2 = d;
+    for (int k = 0; k < 2; k++) {
+        A2avDev& x = k == 0 ? d : pl->dev2;
+        char* b = w + k * ws;
+        x.seg_end = (uint64_t*)b;
+        x.seg_src = x.seg_end + PP;
+        x.row_bytes = x.seg_src + PP;
+        x.chunk_off = x.row_bytes + P;
+        x.total_bytes = (int64_t*)(x.chunk_off + P + 1);
+        x.plan_done = (uint32_t*)(b + ws - 128);
+    }
     for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
-    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4, 0};
+    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4, 0, 1};
     if (tune) {
         if (tune->block_threads > 0) t.block_threads = tune->block_threads;
         if (tune->blocks_per_cu >= 0) t.blocks_per_cu = tune->blocks_per_cu;
@@ -978,6 +1034,7 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
         if (tune->nontemporal >= 0) t.nontemporal = tune->nontemporal;
         if (tune->chunk_rounds > 0) t.chunk_rounds = tune->chunk_rounds;
         if (tune->defer_straddle >= 0) t.defer_straddle = tune->defer_straddle;
+        if (tune->pipeline >= 0) t.pipeline = tune->pipeline;
     }
     int rc = a2av_configure(pl, t);
     if (rc != GPUMPI_OK) {
@@ -1007,35 +1064,76 @@ int gpumpi_a2av_execute(gpumpi_a2av_plan* pl, void* stream) {
 int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* pl, int32_t steps, void* stream) {
     if (!pl || steps <= 0) return GPUMPI_E_ARG;
     hipStream_t s = (hipStream_t)stream;
-    if (!pl->graph || pl->graph_steps != steps) {
-        a2av_drop_graph(pl);
+    A2avGraph* g = nullptr;
+    for (A2avGraph& x : pl->graphs)
+        if (x.steps == steps) g = &x;
+    if (!g) {
         hipStream_t cs;
         if (hipStreamCreate(&cs) != hipSuccess) return GPUMPI_E_HIP;
-        if (hipStreamSynchronize(s) != hipSuccess) {
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
             hipStreamDestroy(cs);
             return GPUMPI_E_HIP;
         }
-        if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
-            hipStreamDestroy(cs);
-            return GPUMPI_E_HIP;
-        }
-        int rc = GPUMPI_OK;
-        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) rc = a2av_launch(pl, cs, false);
-        hipGraph_t g = nullptr;
-        hipError_t ee = hipStreamEndCapture(cs, &g);
+        const int rc = a2av_launch_steps(pl, cs, steps);
+        A2avGraph ng;
+        ng.steps = steps;
+        hipError_t ee = hipStreamEndCapture(cs, &ng.def);
         hipStreamDestroy(cs);
         if (rc != GPUMPI_OK || ee != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+            if (ng.def) hipGraphDestroy(ng.def);
             return GPUMPI_E_HIP;
         }
-        pl->graph_def = g;
-        if (hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0) != hipSuccess) {
-            a2av_drop_graph(pl);
+        if (hipGraphInstantiate(&ng.exec, ng.def, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(ng.def);
             return GPUMPI_E_HIP;
         }
-        pl->graph_steps = steps;
+        pl->graphs.push_back(ng);
+        g = &pl->graphs.back();
     }
-    return hipGraphLaunch(pl->graph, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+    pl->last_ws = pl->tune.pipeline ? (steps - 1) & 1 : 0;
+    return hipGraphLaunch(g->exec, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+// Durations of the kernels of `steps` pipelined steps launched directly (not from a
+// graph), with HIP events on the launch stream: us[0] = plan + scan of the first step,
+// us[1] = average copy kernel (each but the last also planning the next step), us[2] =
+// the whole sequence per step.
+int gpumpi_a2av_pipeline_times(gpumpi_a2av_plan* pl, int32_t steps, void* stream, double* us) {
+    if (!pl || steps < 2 || !us) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<hipEvent_t> ev((size_t)steps + 2);
+    for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return GPUMPI_E_HIP;
+    int rc = GPUMPI_OK;
+    A2avDev& d = pl->dev;
+    void* kargs1[] = {(void*)&d};
+    hipEventRecord(ev[0], s);
+    if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(d.P), dim3(256), kargs1, 0, s) != hipSuccess ||
+        hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess)
+        rc = GPUMPI_E_HIP;
+    hipEventRecord(ev[1], s);
+    for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
+        rc = a2av_launch_copy(pl, s, i & 1, i + 1 < steps);
+        hipEventRecord(ev[(size_t)i + 2], s);
+    }
+    if (rc == GPUMPI_OK && hipEventSynchronize(ev.back()) == hipSuccess) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev[0], ev[1]);
+        us[0] = 1000.0 * ms;
+        double copy = 0;
+        for (int i = 0; i < steps; i++) {
+            hipEventElapsedTime(&ms, ev[(size_t)i + 1], ev[(size_t)i + 2]);
+            copy += 1000.0 * ms;
+        }
+        us[1] = copy / steps;
+        hipEventElapsedTime(&ms, ev[0], ev.back());
+        us[2] = 1000.0 * ms / steps;
+    } else if (rc == GPUMPI_OK) {
+        rc = GPUMPI_E_HIP;
+    }
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    return rc;
 }
 
 int gpumpi_a2av_phase_times(gpumpi_a2av_plan* pl, int32_t enable, double* us) {
@@ -1085,6 +1183,7 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
     const int nts[] = {1, 0};
     const int defers[] = {0, 1};
     gpumpi_a2av_tune best = pl->tune;
+    const int32_t pipeline = pl->tune.pipeline;
     double best_t = 1e30;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
@@ -1094,7 +1193,7 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
         for (int U : us_)
             for (int R : rs)
             for (int nt : nts) {
-                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt, R, defer};
+                gpumpi_a2av_tune t{kA2avCopyThreads, bpc, U, nt, R, defer, pipeline};
                 if (a2av_configure(pl, t) != GPUMPI_OK) continue;
                 a2av_launch(pl, s, false);  // warm-up
                 const int reps = 5;
@@ -1614,6 +1713,138 @@ int64_t gpumpi_job_heap_bytes(const gpumpi_job* job) {
     if (!job) return -1;
     progs::JobDesc d{job->id, job->seed, job->type, job->ranks, job->size};
     return progs::job_bytes(d);
+}
+
+}  // extern "C"
+
+// ==================================================================== batched Reduce / Allreduce
+struct gpumpi_reduce_plan {
+    ReduceDev dev{};
+    const void* kernel = nullptr;
+    int grid = 0;
+    std::vector<A2avGraph> graphs;
+};
+
+static int reduce_launch(gpumpi_reduce_plan* pl, hipStream_t s) {
+    void* kargs[] = {(void*)&pl->dev};
+    return hipLaunchKernel(pl->kernel, dim3(pl->grid), dim3(kReduceThreads), kargs, 0, s) == hipSuccess
+               ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+extern "C" {
+
+int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_t op, int32_t root,
+                              void* const* sendbufs, void* const* recvbufs,
+                              gpumpi_reduce_plan** out) {
+    if (!out || P <= 0 || count < 0 || !sendbufs || !recvbufs || root < -1 || root >= P)
+        return GPUMPI_E_ARG;
+    *out = nullptr;
+    const int es = type_size(datatype);
+    const void* k = es > 0 ? batched_reduce_kernel_for(datatype, op) : nullptr;
+    if (!k) {
+        g_host_error = "batched reduction: the pair (datatype " + std::to_string(datatype) +
+                       ", operator " + std::to_string(op) + ") is not on the batched path "
+                       "(int32/uint32/int64/uint64/float/double with MAX, MIN, SUM, PROD); "
+                       "MPI_Reduce on the rank grid takes every legal pair";
+        return GPUMPI_E_ARG;
+    }
+    int32_t cus = 0;
+    if (gpumpi_device_info(&cus, nullptr, nullptr) != GPUMPI_OK) return GPUMPI_E_HIP;
+    gpumpi_reduce_plan* pl = new gpumpi_reduce_plan();
+    pl->kernel = k;
+    const uint64_t bytes = (uint64_t)count * (uint64_t)es;
+    const uint64_t written = bytes * (root < 0 ? (uint64_t)P : 1u);
+    pl->dev.P = P;
+    pl->dev.root = root;
+    pl->dev.count = count;
+    pl->dev.nt = written >= (64ull << 20) ? 1 : 0;  // outputs larger than the L2s stream past them
+    pl->dev.sendbufs = (const char*)sendbufs;
+    pl->dev.recvbufs = (const char*)recvbufs;
+    const uint64_t nvec = bytes / 16;
+    uint64_t g = (nvec + kReduceThreads - 1) / kReduceThreads;
+    if (g < 1) g = 1;
+    if (g > (1u << 20)) g = 1u << 20;
+    pl->grid = (int)g;
+    *out = pl;
+    return GPUMPI_OK;
+}
+
+void gpumpi_reduce_plan_destroy(gpumpi_reduce_plan* pl) {
+    if (!pl) return;
+    hipDeviceSynchronize();
+    for (A2avGraph& g : pl->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.def) hipGraphDestroy(g.def);
+    }
+    delete pl;
+}
+
+int gpumpi_reduce_execute(gpumpi_reduce_plan* pl, void* stream) {
+    if (!pl) return GPUMPI_E_ARG;
+    return reduce_launch(pl, (hipStream_t)stream);
+}
+
+int gpumpi_reduce_graph_replay(gpumpi_reduce_plan* pl, int32_t steps, void* stream) {
+    if (!pl || steps <= 0) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    A2avGraph* g = nullptr;
+    for (A2avGraph& x : pl->graphs)
+        if (x.steps == steps) g = &x;
+    if (!g) {
+        hipStream_t cs;
+        if (hipStreamCreate(&cs) != hipSuccess) return GPUMPI_E_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            hipStreamDestroy(cs);
+            return GPUMPI_E_HIP;
+        }
+        int rc = GPUMPI_OK;
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) rc = reduce_launch(pl, cs);
+        A2avGraph ng;
+        ng.steps = steps;
+        hipError_t ee = hipStreamEndCapture(cs, &ng.def);
+        hipStreamDestroy(cs);
+        if (rc != GPUMPI_OK || ee != hipSuccess) {
+            if (ng.def) hipGraphDestroy(ng.def);
+            return GPUMPI_E_HIP;
+        }
+        if (hipGraphInstantiate(&ng.exec, ng.def, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(ng.def);
+            return GPUMPI_E_HIP;
+        }
+        pl->graphs.push_back(ng);
+        g = &pl->graphs.back();
+    }
+    return hipGraphLaunch(g->exec, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_reduce_kernel_time(gpumpi_reduce_plan* pl, int32_t reps, void* stream, double* us) {
+    if (!pl || reps < 1 || !us) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return GPUMPI_E_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return GPUMPI_E_HIP;
+    }
+    double acc = 0;
+    int rc = GPUMPI_OK;
+    for (int i = 0; i < reps && rc == GPUMPI_OK; i++) {
+        hipEventRecord(e0, s);
+        rc = reduce_launch(pl, s);
+        hipEventRecord(e1, s);
+        if (rc == GPUMPI_OK && hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            acc += 1000.0 * ms;
+        } else {
+            rc = GPUMPI_E_HIP;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc == GPUMPI_OK) *us = acc / reps;
+    return rc;
 }
 
 }  // extern "C"

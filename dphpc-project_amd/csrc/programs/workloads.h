@@ -194,6 +194,8 @@ struct CollArgs {
     int64_t in_stride;
     int64_t out_stride;
     double* seconds;     // [P]
+    int32_t no_fill;     // 1: the inputs are already in place (timing one call per launch)
+    int32_t pad;
 };
 
 __device__ void fill_input(char* p, int dtype, int64_t n, uint64_t seed, int rank) {
@@ -217,7 +219,7 @@ __device__ int w_coll(int, char**) {
     const int64_t n = a.count;
     int64_t nin = n;
     if (a.kind == 6) nin = rank == a.root ? n * P : 0;
-    fill_input(in, a.dtype, nin, a.seed, rank);
+    if (!a.no_fill) fill_input(in, a.dtype, nin, a.seed, rank);
     double t0 = 0;
     for (int it = -a.warmup; it < a.iters; it++) {
         if (it == 0) {

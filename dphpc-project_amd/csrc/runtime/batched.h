@@ -66,6 +66,7 @@ struct A2avDev {
     uint64_t* row_bytes;    // [P]
     uint64_t* chunk_off;    // [P+1] exclusive prefix of chunks per row; [P] = total chunks
     int64_t* total_bytes;   // [1]
+    uint32_t* plan_done;    // [1] plan workgroups finished (fused plan; reset by the last one)
 };
 
 __device__ __forceinline__ int64_t wave_excl_scan(int64_t v, int64_t* total) {
@@ -99,12 +100,10 @@ __device__ __forceinline__ int64_t wave_search_le(const uint64_t* a, int64_t n, 
 // kept in registers: the quarters' count totals meet in LDS to give each quarter's
 // starting displacement, then each wave scans and writes.  Sources past 256 per quarter
 // are summed in a first pass and reloaded in the second.
-__global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
-    __shared__ int64_t s_tot[4];
+__device__ __forceinline__ void a2av_plan_row(const A2avDev& p, int64_t d, int64_t* s_tot) {
     const int P = p.P;
     const int l = lane();
     const int w = threadIdx.x >> 6;
-    const int64_t d = blockIdx.x;
     const int Q = (((P + 3) / 4) + kWave - 1) / kWave * kWave;
     const int sb = w * Q;
     const int se = sb + Q < P ? sb + Q : P;
@@ -161,16 +160,22 @@ __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
     if (threadIdx.x == 0) p.row_bytes[d] = (uint64_t)row * (uint64_t)p.elem;
 }
 
+__global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
+    __shared__ int64_t s_tot[4];
+    a2av_plan_row(p, blockIdx.x, s_tot);
+}
+
 // ------------------------------------------------------------------------------ scan
-__global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
-    __shared__ uint64_t part[16];
-    __shared__ int64_t bpart[16];
+// Run by one workgroup of NT threads (NT a multiple of 64, at most 1024).
+template <int NT>
+__device__ __forceinline__ void a2av_scan_rows(const A2avDev& p, uint64_t* part, int64_t* bpart) {
+    constexpr int NW = NT / kWave;
     const int P = p.P;
     const int w = threadIdx.x >> 6;
     const uint64_t C = p.chunk_lines;
     uint64_t carry = 0;
     int64_t bytes = 0;
-    for (int base = 0; base < P; base += 1024) {
+    for (int base = 0; base < P; base += NT) {
         const int d = base + (int)threadIdx.x;
         uint64_t ch = 0;
         int64_t nb = 0;
@@ -186,7 +191,7 @@ __global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
         if (lane() == 0) bpart[w] = bsum;
         __syncthreads();
         uint64_t before = 0, total = 0;
-        for (int k = 0; k < 16; k++) {
+        for (int k = 0; k < NW; k++) {
             if (k < w) before += part[k];
             total += part[k];
             bytes += (threadIdx.x == 0) ? bpart[k] : 0;
@@ -199,6 +204,39 @@ __global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
         p.chunk_off[P] = carry;
         *p.total_bytes = bytes;
     }
+}
+
+__global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
+    __shared__ uint64_t part[16];
+    __shared__ int64_t bpart[16];
+    a2av_scan_rows<1024>(p, part, bpart);
+}
+
+// Plan of the NEXT step run by the first P workgroups of a copy kernel (the step's only
+// global dependency -- the receive displacements -- is then resolved while the current
+// step streams, instead of in two kernels of its own on the critical path).  The
+// workgroup that finishes last computes the chunk prefix: every workgroup releases its
+// tables at agent scope before it counts itself done, and the last one acquires them.
+// The tables live in the other of two workspaces, so the copy running beside them is
+// not disturbed.
+__device__ void a2av_plan_fused(const A2avDev& pn, int64_t d) {
+    __shared__ int64_t s_tot[4];
+    __shared__ uint64_t s_part[4];
+    __shared__ int64_t s_bpart[4];
+    __shared__ int32_t s_last;
+    a2av_plan_row(pn, d, s_tot);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t old = __hip_atomic_fetch_add(pn.plan_done, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        s_last = old == (uint32_t)pn.P - 1u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    a2av_scan_rows<kA2avCopyThreads>(pn, s_part, s_bpart);
+    if (threadIdx.x == 0) *pn.plan_done = 0u;  // next used two kernels later
 }
 
 // ------------------------------------------------------------------------------ straddle
@@ -397,8 +435,10 @@ __device__ void a2av_boundary_line(const A2avDev& p, int64_t d, uint64_t rbase, 
     }
 }
 
-template <int U, bool NT, bool DEFER>
-__global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) {
+// FUSE: the first pn.P workgroups plan the next step into pn (a2av_plan_fused); the copy
+// workgroups follow them.
+template <int U, bool NT, bool DEFER, bool FUSE>
+__global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p, A2avDev pn) {
     __shared__ int64_t s_st[kA2avWindow];
     __shared__ int64_t s_en[kA2avWindow];
     __shared__ uint64_t s_D[kA2avWindow];
@@ -407,19 +447,28 @@ __global__ void __launch_bounds__(kA2avCopyThreads) a2av_copy_kernel(A2avDev p) 
     const int tid = threadIdx.x;
     const int l = lane();
     const uint64_t C = p.chunk_lines;
+    uint32_t bid = blockIdx.x, nblk = gridDim.x;
+    if constexpr (FUSE) {
+        if (bid < (uint32_t)pn.P) {  // block-uniform
+            a2av_plan_fused(pn, bid);
+            return;
+        }
+        bid -= (uint32_t)pn.P;
+        nblk -= (uint32_t)pn.P;
+    }
     uint32_t sb = 0;
     if constexpr (DEFER) {
         sb = (uint32_t)p.straddle_blocks;
-        if (blockIdx.x < sb) {  // block-uniform
+        if (bid < sb) {  // block-uniform
             const uint64_t nseg = (uint64_t)P * (uint64_t)P;
-            for (uint64_t g = (uint64_t)blockIdx.x * kA2avCopyThreads + tid; g < nseg;
+            for (uint64_t g = (uint64_t)bid * kA2avCopyThreads + tid; g < nseg;
                  g += (uint64_t)sb * kA2avCopyThreads)
                 a2av_straddle_line(p, g);
             return;
         }
     }
     const uint64_t total = p.chunk_off[P];
-    for (uint64_t ch = blockIdx.x - sb; ch < total; ch += gridDim.x - sb) {
+    for (uint64_t ch = bid - sb; ch < total; ch += nblk - sb) {
         // ---- the chunk's row and line range
         if (tid < kWave) {
             const int64_t d = wave_search_le(p.chunk_off, P, ch);

@@ -23,6 +23,7 @@
 #pragma once
 
 #include "comm.h"
+#include "fold.h"
 #include "ops.h"
 
 namespace gpumpi {
@@ -90,8 +91,11 @@ struct ReduceFunctor {
     const CommImpl* c;
     int64_t lo, hi;
     int root;  // -1: all ranks receive
+    bool nt;   // non-temporal result stores (large buffers)
+    // Out of line, one function per (type, operator): inlined into coll_reduce, the fold's
+    // load ring pushed the collective past 256 registers per lane.
     template <class T, int OP>
-    __device__ void run() {
+    __device__ __attribute__((noinline)) void run() {
         const int P = c->size;
         const CollSlot* slots = c->slots;
         if constexpr (sizeof(T) <= 8) {
@@ -129,26 +133,26 @@ struct ReduceFunctor {
                 return;
             }
         }
-        for (int64_t i = lo + lane(); i < hi; i += kWave) {
-            T acc = ((const T*)slots[0].sbuf)[i];
-            int r = 1;
-            for (; r + 3 < P; r += 4) {
-                T x1 = ((const T*)slots[r].sbuf)[i];
-                T x2 = ((const T*)slots[r + 1].sbuf)[i];
-                T x3 = ((const T*)slots[r + 2].sbuf)[i];
-                T x4 = ((const T*)slots[r + 3].sbuf)[i];
-                acc = combine<T, OP>(x1, acc);
-                acc = combine<T, OP>(x2, acc);
-                acc = combine<T, OP>(x3, acc);
-                acc = combine<T, OP>(x4, acc);
-            }
-            for (; r < P; r++) acc = combine<T, OP>(((const T*)slots[r].sbuf)[i], acc);
-            if (root >= 0) {
-                ((T*)slots[root].rbuf)[i] = acc;
-            } else {
-                for (int d = 0; d < P; d++) ((T*)slots[d].rbuf)[i] = acc;
+        // The slice in 16-byte columns, one per lane, each folded over the P inputs in rank
+        // order with kFoldDepth loads in flight (fold.h); the elements past the last whole
+        // column one per lane.  1- and 2-byte types go element by element (a 16-byte column
+        // of them unpacks into 8-16 values per lane).
+        const PtrTable src{(const char*)&slots[0].sbuf, (int64_t)sizeof(CollSlot)};
+        const PtrTable dst{(const char*)&slots[0].rbuf, (int64_t)sizeof(CollSlot)};
+        const uint64_t one = root >= 0 ? (uint64_t)(uintptr_t)slots[root].rbuf : 0;
+        const int64_t b0 = lo * (int64_t)sizeof(T);
+        const int64_t nv = sizeof(T) >= 4 ? (hi - lo) * (int64_t)sizeof(T) / 16 : 0;
+        for (int64_t v0 = 0; v0 < nv; v0 += kWave) {
+            const int64_t v = v0 + lane();
+            const bool valid = v < nv;
+            const uint64_t off = (uint64_t)(b0 + 16 * v);
+            if constexpr (sizeof(T) >= 4) {
+                const u32x4 acc = fold_column<T, OP>(src, P, off, valid);
+                if (nt) fold_store<true>(dst, P, one, off, valid, acc);
+                else fold_store<false>(dst, P, one, off, valid, acc);
             }
         }
+        fold_scalar_tail<T, OP>(src, P, dst, P, one, lo + nv * 16 / (int64_t)sizeof(T), hi);
     }
 };
 
@@ -185,7 +189,7 @@ __device__ int coll_reduce(const void* sendbuf, void* recvbuf, int64_t count, in
         // to the closing barrier, which keeps every buffer alive until the reads are done
         if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
         if (op->builtin >= 0) {
-            ReduceFunctor f{c, lo, hi, root};
+            ReduceFunctor f{c, lo, hi, root, bytes >= kNontemporalBytes};
             dispatch_op(dt, op->builtin, f);
         } else {
             // user operator: fold on lane 0 into the first destination, then replicate

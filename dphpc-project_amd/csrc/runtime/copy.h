@@ -45,9 +45,9 @@ __device__ void wave_copy_global(char* d, const char* s, uint64_t n, bool nt) {
     const uint64_t nvec = n >> 4;
     const uint64_t sa = (uint64_t)(uintptr_t)s, dst = (uint64_t)(uintptr_t)d;
     uint64_t base = 0;
-    // bulk: whole rounds of kBulk vectors per lane, every load issued before the first
-    // store and none behind a bounds check (a lone wavefront's copy rate is set by the
-    // bytes it keeps in flight)
+    // bulk: whole rounds of kBulk vectors per lane (16 KiB per wave), every load issued
+    // before the first store and none behind a bounds check (a lone wavefront's copy rate
+    // is set by the bytes it keeps in flight)
     constexpr int kBulk = 8;
     const uint64_t full = nvec - nvec % ((uint64_t)kWave * kBulk);
     for (; base < full; base += (uint64_t)kWave * kBulk) {

@@ -1,0 +1,122 @@
+// Element-wise reduction of P rank buffers in the reference's fold order
+// (acc = in_0; acc = op(in_r, acc) for r = 1..P-1, gpu_mpi/mpi.cu:209-225, operator bodies
+// operators.cu:73-151), one 16-byte column per lane.
+//
+// Every element keeps its own left-to-right fold, so results are bit-exact with the
+// reference's generic path for every type, FP64 included; the parallelism is over
+// elements only.  A lane's column is loaded from the P sources in rank order with
+// kFoldDepth loads in flight (a ring of 16-byte registers, refilled DEPTH ranks ahead),
+// which keeps a wave's 64 columns (1 KiB per rank) streaming from HBM instead of waiting
+// one load at a time.  Source and destination addresses come from tables of per-rank
+// pointers (a stride apart); a wave loads 64 of them at once into its lanes and reads
+// each back with readlane.
+#pragma once
+
+#include "ops.h"
+#include "copy.h"
+
+namespace gpumpi {
+
+constexpr int kFoldDepth = 16;
+
+// Per-rank pointer tables: entry r at base + r * stride (a uint64_t address).
+struct PtrTable {
+    const char* base;
+    int64_t stride;
+    __device__ __forceinline__ uint64_t lane_chunk(int r0, int P) const {
+        const int r = r0 + lane();
+        return r < P ? *(const uint64_t*)(base + (int64_t)r * stride) : 0;
+    }
+};
+
+// acc = op(x, acc) on the V = 16 / sizeof(T) elements of a 16-byte vector.
+template <class T, int OP>
+__device__ __forceinline__ u32x4 fold_vec_op(u32x4 x, u32x4 acc) {
+    constexpr int V = 16 / (int)sizeof(T);
+    T a[V], b[V];
+    __builtin_memcpy(a, &x, 16);
+    __builtin_memcpy(b, &acc, 16);
+#pragma unroll
+    for (int i = 0; i < V; i++) b[i] = combine<T, OP>(a[i], b[i]);
+    u32x4 out;
+    __builtin_memcpy(&out, b, 16);
+    return out;
+}
+
+// Folds the 16-byte column at byte offset `off` of every source (ranks 0..P-1 in order)
+// and returns it; `valid` lanes only load.  Wave-uniform P; all lanes must call.
+template <class T, int OP>
+__device__ __forceinline__ u32x4 fold_column(const PtrTable& src, int P, uint64_t off, bool valid) {
+    constexpr int D = kFoldDepth;
+    u32x4 ring[D];
+    uint64_t pc = src.lane_chunk(0, P);
+    uint64_t pn = P > kWave ? src.lane_chunk(kWave, P) : 0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        ring[k] = u32x4{0u, 0u, 0u, 0u};
+        if (k < P && valid) ring[k] = gload16(readlane64(pc, k) + off);
+    }
+    u32x4 acc = u32x4{0u, 0u, 0u, 0u};
+    for (int r0 = 0; r0 < P; r0 += kWave) {
+        for (int j0 = 0; j0 < kWave; j0 += D) {
+            if (r0 + j0 >= P) break;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const int r = r0 + j0 + k;
+                if (r < P) {
+                    const u32x4 x = ring[k];
+                    const int jn = j0 + k + D;  // chunk-relative index of rank r + D
+                    if (r + D < P && valid) {
+                        const uint64_t p = jn < kWave ? readlane64(pc, jn) : readlane64(pn, jn - kWave);
+                        ring[k] = gload16(p + off);
+                    }
+                    acc = r == 0 ? x : fold_vec_op<T, OP>(x, acc);
+                }
+            }
+        }
+        pc = pn;
+        pn = r0 + 2 * kWave < P ? src.lane_chunk(r0 + 2 * kWave, P) : 0;
+    }
+    return acc;
+}
+
+// Stores a lane's folded column to every destination of the table (ndst of them), or to
+// one destination address when `one` is non-zero.
+template <bool NT>
+__device__ __forceinline__ void fold_store(const PtrTable& dst, int ndst, uint64_t one, uint64_t off,
+                                           bool valid, u32x4 v) {
+    if (one) {
+        if (valid) gstore16<NT>(one + off, v);
+        return;
+    }
+    for (int d0 = 0; d0 < ndst; d0 += kWave) {
+        const uint64_t pd = dst.lane_chunk(d0, ndst);
+        const int n = ndst - d0 < kWave ? ndst - d0 : kWave;
+        for (int j = 0; j < n; j++)
+            if (valid) gstore16<NT>(readlane64(pd, j) + off, v);
+    }
+}
+
+// Elements [e0, e1) of the P sources, folded one element per lane (the part of a range
+// that does not fill a 16-byte column).
+template <class T, int OP>
+__device__ __forceinline__ void fold_scalar_tail(const PtrTable& src, int P, const PtrTable& dst,
+                                                 int ndst, uint64_t one, int64_t e0, int64_t e1) {
+    for (int64_t e = e0 + lane(); e < e1; e += kWave) {
+        T acc = *(const T*)(*(const uint64_t*)src.base + e * (int64_t)sizeof(T));
+        for (int r = 1; r < P; r++) {
+            const uint64_t p = *(const uint64_t*)(src.base + (int64_t)r * src.stride);
+            acc = combine<T, OP>(*(const T*)(p + e * (int64_t)sizeof(T)), acc);
+        }
+        if (one) {
+            *(T*)(one + e * (int64_t)sizeof(T)) = acc;
+        } else {
+            for (int d = 0; d < ndst; d++) {
+                const uint64_t p = *(const uint64_t*)(dst.base + (int64_t)d * dst.stride);
+                *(T*)(p + e * (int64_t)sizeof(T)) = acc;
+            }
+        }
+    }
+}
+
+}  // namespace gpumpi

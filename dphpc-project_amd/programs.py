@@ -25,7 +25,7 @@ class CollArgs(ctypes.Structure):
                 ("op", ctypes.c_int32), ("root", ctypes.c_int32), ("count", ctypes.c_int64),
                 ("iters", ctypes.c_int32), ("warmup", ctypes.c_int32), ("inbase", _P),
                 ("outbase", _P), ("in_stride", ctypes.c_int64), ("out_stride", ctypes.c_int64),
-                ("seconds", _P)]
+                ("seconds", _P), ("no_fill", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class ChurnArgs(ctypes.Structure):

@@ -137,6 +137,10 @@ typedef struct gpumpi_a2av_tune {
     int32_t defer_straddle;     /* 1 = lines holding several segments are assembled by one
                                    thread per segment in leading copy blocks (pays for short
                                    segments), 0 = by the copy loop, -1 = default (0) */
+    int32_t pipeline;           /* graph replays: 1 = each step's copy kernel also plans the
+                                   next step (into a second workspace), so plan and scan
+                                   leave the critical path; 0 = plan, scan, copy kernels per
+                                   step; -1 = default (1) */
 } gpumpi_a2av_tune;
 
 int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,
@@ -152,6 +156,10 @@ int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* plan, int32_t steps_per_graph, vo
 /* Sweep launch shapes on the live buffers, keep the fastest; returns the chosen shape. */
 int gpumpi_a2av_autotune(gpumpi_a2av_plan* plan, void* stream, gpumpi_a2av_tune* chosen,
                          double* best_us);
+/* Kernel durations of `steps` (>= 2) pipelined steps launched directly, timed with HIP
+ * events on `stream`: us[0] = the first step's plan + scan, us[1] = the average copy kernel
+ * (each but the last also planning the next step), us[2] = the sequence's time per step. */
+int gpumpi_a2av_pipeline_times(gpumpi_a2av_plan* plan, int32_t steps, void* stream, double* us);
 /* Per-kernel durations of a step (profiling): enable = 1 makes gpumpi_a2av_execute record
  * HIP events around its kernels, 0 stops it, -1 reads us[3] = {plan, scan, copy} in
  * microseconds for the last executed step (graph replays are not timed). */
@@ -160,6 +168,32 @@ int gpumpi_a2av_phase_times(gpumpi_a2av_plan* plan, int32_t enable, double* us);
 int gpumpi_a2av_get_tune(gpumpi_a2av_plan* plan, gpumpi_a2av_tune* tune, int32_t* grid);
 /* Total payload bytes moved by the last completed step (reads them back from the device). */
 int gpumpi_a2av_payload_bytes(gpumpi_a2av_plan* plan, int64_t* bytes);
+
+/* ------------------------------------------------------------------------------------
+ * Batched MPI_Reduce / MPI_Allreduce: the reduction of one step over P ranks as one
+ * kernel (replayable from a hipGraph), for callers that hold every rank's buffer on the
+ * GPU.  Replaces the call MPI_Reduce(sendbuf, recvbuf, count, datatype, op, root, comm)
+ * (gpu_libs/gpu_mpi/mpi.cu:148-235; root = -1: MPI_Allreduce, mpi.cu:276-282) made by
+ * all P ranks of a step.  Every element is folded in the reference's rank order
+ * (inout = op(in_r, inout), r = 1..P-1, mpi.cu:209-225), so FP64 results are the rank
+ * grid's bit for bit.  sendbufs / recvbufs: device arrays of P device pointers (recvbufs
+ * entries other than the root's are not written by MPI_Reduce); count elements per rank.
+ * datatype is an MPI_Datatype, op a built-in operator index (MPI_MAX = 0, MPI_MIN = 1,
+ * MPI_SUM = 2, MPI_PROD = 3).  The batched path takes int32, uint32, int64, uint64, float
+ * and double with those four; any other pair returns GPUMPI_E_ARG (the rank grid's
+ * MPI_Reduce takes every legal pair).
+ * ------------------------------------------------------------------------------------ */
+typedef struct gpumpi_reduce_plan gpumpi_reduce_plan;
+int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_t op, int32_t root,
+                              void* const* sendbufs, void* const* recvbufs,
+                              gpumpi_reduce_plan** out);
+void gpumpi_reduce_plan_destroy(gpumpi_reduce_plan* plan);
+/* One step on `stream` (asynchronous). */
+int gpumpi_reduce_execute(gpumpi_reduce_plan* plan, void* stream);
+/* Capture `steps` steps into a hipGraph once per step count, then replay it. */
+int gpumpi_reduce_graph_replay(gpumpi_reduce_plan* plan, int32_t steps, void* stream);
+/* Average duration of the kernel over `reps` launches, HIP events on `stream` (us). */
+int gpumpi_reduce_kernel_time(gpumpi_reduce_plan* plan, int32_t reps, void* stream, double* us);
 
 /* ------------------------------------------------------------------------------------
  * Batched particle step: the particle variant of the exchange (tally, place and scatter

@@ -128,6 +128,96 @@ def test_a2av_graph_and_autotune(pkg, oracle, torch_gpu):
         assert np.array_equal(got[d], exp[d][:got[d].size])
 
 
+@pytest.mark.parametrize("defer", [0, 1])
+@pytest.mark.parametrize("pipeline", [1, 0])
+@pytest.mark.parametrize("steps", [1, 2, 3, 10, 11])
+def test_a2av_pipelined_graphs(steps, pipeline, defer, pkg, oracle, torch_gpu):
+    """Graphs of 1..11 steps, with each step's copy kernel planning the next step into the
+    other workspace (pipeline 1) or three kernels per step (0): the receive buffers,
+    counts and displacements after the graph are the oracle's."""
+    print(f"[gpu] a2av pipelined graph steps={steps} pipeline={pipeline} defer={defer}", flush=True)
+    rng = np.random.default_rng(100 + steps)
+    send, counts, sdispls = random_case(rng, 24, 4, 900)
+    got, rc, rd, nbytes = run_exchange(pkg, torch_gpu, send, counts, sdispls, 4,
+                                       tune=(0, -1, 0, -1, 0, defer, pipeline), graph_steps=steps)
+    exp, erc, erd = alltoallv(oracle, send, counts, sdispls, 4, counts.sum(axis=0))
+    assert nbytes == int(counts.sum()) * 4
+    assert np.array_equal(rc, erc) and np.array_equal(rd, erd)
+    for d in range(24):
+        assert np.array_equal(got[d], exp[d][:got[d].size]), f"rank {d} payload differs"
+
+
+def test_a2av_pipelined_graph_replans_new_inputs(pkg, oracle, torch_gpu):
+    """A pipelined graph plans its first step from the inputs as they are at launch: new
+    counts, displacements and payloads written between two replays give the new
+    exchange (buffers sized for both)."""
+    torch = torch_gpu
+    dev = torch.device("cuda")
+    P, elem = 20, 4
+    rng = np.random.default_rng(77)
+    cases = [random_case(rng, P, elem, 500), random_case(rng, P, elem, 500)]
+    cap_s = max(max(b.size for b in c[0]) for c in cases)
+    cap_r = max(int(c[1].sum(axis=0).max()) for c in cases) * elem
+    d_send = [torch.zeros(cap_s, dtype=torch.uint8, device=dev) for _ in range(P)]
+    d_recv = [torch.zeros(cap_r + 16, dtype=torch.uint8, device=dev) for _ in range(P)]
+    sptr = torch.tensor([t.data_ptr() for t in d_send], dtype=torch.int64, device=dev)
+    rptr = torch.tensor([t.data_ptr() for t in d_recv], dtype=torch.int64, device=dev)
+    d_sc = torch.zeros((P, P), dtype=torch.int32, device=dev)
+    d_sd = torch.zeros_like(d_sc)
+    d_rc = torch.zeros_like(d_sc)
+    d_rd = torch.zeros_like(d_sc)
+    plan = pkg.A2avPlan(P, elem, sptr.data_ptr(), d_sc.data_ptr(), d_sd.data_ptr(),
+                        rptr.data_ptr(), d_rc.data_ptr(), d_rd.data_ptr(),
+                        tune=(0, -1, 0, -1, 0, 0, 1))
+    try:
+        for send, counts, sdispls in cases + cases[:1]:
+            for s_ in range(P):
+                d_send[s_][:send[s_].size].copy_(torch.from_numpy(send[s_]))
+            d_sc.copy_(torch.from_numpy(counts))
+            d_sd.copy_(torch.from_numpy(sdispls))
+            torch.cuda.synchronize()
+            plan.graph_replay(5)
+            torch.cuda.synchronize()
+            exp, erc, erd = alltoallv(oracle, send, counts, sdispls, elem, counts.sum(axis=0))
+            assert np.array_equal(d_rc.cpu().numpy(), erc)
+            assert np.array_equal(d_rd.cpu().numpy(), erd)
+            for d in range(P):
+                n = int(counts[:, d].sum()) * elem
+                assert np.array_equal(d_recv[d][:n].cpu().numpy(), exp[d][:n]), d
+    finally:
+        plan.close()
+
+
+def test_a2av_pipeline_times(pkg, oracle, torch_gpu):
+    torch = torch_gpu
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(9)
+    send, counts, sdispls = random_case(rng, 16, 4, 3000)
+    P = 16
+    d_send = [torch.from_numpy(b.copy()).to(dev) for b in send]
+    rtot = counts.sum(axis=0)
+    d_recv = [torch.zeros(int(n) * 4 + 16, dtype=torch.uint8, device=dev) for n in rtot]
+    sptr = torch.tensor([t.data_ptr() for t in d_send], dtype=torch.int64, device=dev)
+    rptr = torch.tensor([t.data_ptr() for t in d_recv], dtype=torch.int64, device=dev)
+    d_sc = torch.from_numpy(counts).to(dev)
+    d_sd = torch.from_numpy(sdispls).to(dev)
+    d_rc = torch.zeros_like(d_sc)
+    d_rd = torch.zeros_like(d_sc)
+    plan = pkg.A2avPlan(P, 4, sptr.data_ptr(), d_sc.data_ptr(), d_sd.data_ptr(),
+                        rptr.data_ptr(), d_rc.data_ptr(), d_rd.data_ptr())
+    try:
+        t = plan.pipeline_times(6)
+        assert t["plan_scan_us"] > 0 and t["copy_us"] > 0 and t["step_us"] > 0
+        torch.cuda.synchronize()
+        assert plan.payload_bytes() == int(counts.sum()) * 4
+        exp, _, _ = alltoallv(oracle, send, counts, sdispls, 4, rtot)
+        for d in range(P):
+            n = int(rtot[d]) * 4
+            assert np.array_equal(d_recv[d][:n].cpu().numpy(), exp[d][:n]), d
+    finally:
+        plan.close()
+
+
 @pytest.mark.parametrize("dist", ["uniform", "zipf"])
 def test_bucket_exchange_small(dist, pkg, oracle, torch_gpu):
     """NPB-style bucket exchange of the workload keys at a reduced size."""

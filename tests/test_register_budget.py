@@ -1,0 +1,61 @@
+"""Register budget of the rank programs, read from the built gfx950 code object.
+
+The launcher admits a rank grid only if it needs at most half of the wave slots its
+kernel can occupy.  A rank program within 256 VGPRs + AGPRs per lane holds two waves per
+SIMD (2048 on 256 CUs), so 1024-rank grids (the collective sweep, the heap churn, the job
+server) fit; a program above it holds one wave per SIMD and runs at most 512 ranks.  A
+change that pushes a runtime function past the budget shows up here, on the CPU, instead
+of as a refused launch on the GPU."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import PKG_DIR
+
+LLVM = "/opt/rocm/llvm/bin"
+LIB = os.path.join(PKG_DIR, "libgpumpi.so")
+# programs that need more than 256 registers for their own loops, and run at most 512
+# ranks (the NPB IS rank program and the particle exchange, both on 512 ranks)
+HEAVY = {"w_bucket_sort", "w_particles"}
+
+
+def _kernels(tmp_path):
+    tools = [os.path.join(LLVM, t) for t in ("llvm-objcopy", "clang-offload-bundler", "llvm-readelf")]
+    if not all(os.path.exists(t) for t in tools) or not os.path.exists(LIB):
+        pytest.skip("LLVM tools or the built library missing")
+    fat = tmp_path / "fatbin.bin"
+    co = tmp_path / "gfx950.co"
+    subprocess.run([tools[0], "-O", "binary", "--only-section=.hip_fatbin", LIB, str(fat)], check=True)
+    subprocess.run([tools[1], "--unbundle", "--type=o", f"--input={fat}",
+                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
+    notes = subprocess.run([tools[2], "--notes", str(co)], capture_output=True, text=True,
+                           check=True).stdout
+    out = {}
+    for item in re.split(r"\n  - ", notes):
+        m = re.search(r"\.name:\s+(\S+)", item)
+        v = re.search(r"\.vgpr_count:\s+(\d+)", item)
+        if m and v:
+            out[m.group(1)] = int(v.group(1))
+    return out
+
+
+def test_rank_programs_fit_two_waves_per_simd(tmp_path):
+    ks = _kernels(tmp_path)
+    rank = {re.search(r"progs\d+(\w+?)EiPPc", k).group(1): v for k, v in ks.items()
+            if "rank_grid_kernel" in k}
+    assert len(rank) >= 20, rank
+    for name, v in rank.items():
+        limit = 512 if name in HEAVY else 256
+        assert v <= limit, f"{name}: {v} VGPRs + AGPRs per lane (budget {limit})"
+    for name in ("w_coll", "w_churn", "w_serve", "w_job", "w_mcpi", "w_p2p", "w_exchange"):
+        assert rank[name] <= 256, name
+
+
+def test_batched_kernels_are_not_register_bound(tmp_path):
+    ks = _kernels(tmp_path)
+    for k, v in ks.items():
+        if "a2av_copy_kernel" in k or "batched_reduce_kernel" in k:
+            assert v <= 128, f"{k}: {v}"
