@@ -733,7 +733,7 @@ def main():
     # copy kernel that also plans the next step (HIP events on the launch stream, averaged)
     kern = kernel_times(torch, plan, stream, max(5, args.steps // 5))
     pipe = None
-    if tune_used["pipeline"]:
+    if tune_used["pipeline"] in (1, 2):
         pipe = plan.pipeline_times(max(10, args.steps // 5), stream)
         dom_us = pipe["copy_us"]
         dom_name = "a2av_copy_kernel<U, NT, DEFER, FUSE=true> (copy + next step's plan)"
@@ -825,7 +825,7 @@ def main():
     if os.path.exists(tf):
         try:
             tj = json.load(open(tf))
-            if tj.get("kernel_variant", "") == ("fused" if tune_used["pipeline"] else "plain"):
+            if tj.get("kernel_variant", "") == ("fused" if tune_used["pipeline"] in (1, 2) else "plain"):
                 traffic = tj.get("hbm_bytes_per_launch")
                 traffic_src = tj.get("source")
         except Exception:

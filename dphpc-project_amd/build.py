@@ -99,6 +99,62 @@ def build_c_programs(verbose=False):
     return outs
 
 
+# NPB IS and DT, the reference's toolchain tests (toolchain_tests/npb/is_test.sh,
+# dt_test.sh), converted and compiled from the reference's sources where they are present.
+# The sources and the converted text stay in a temporary directory outside the repository;
+# only the executables are kept (tests/programs/_build/npb_is_S, npb_dt_S), so GPU tests
+# can run them and parse the benchmarks' own verification lines.
+NPB = "/root/reference/toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI"
+NPB_COMMON_DEFS = ('#define COMPILETIME "now"\n#define NPBVERSION "3.4.1"\n#define MPICC "gpumpicc"\n'
+                   '#define CFLAGS "-O3"\n#define CLINK "hipcc"\n#define CLINKFLAGS ""\n'
+                   '#define CMPI_LIB "-lgpumpi"\n#define CMPI_INC ""\n')
+NPB_PROGRAMS = [
+    # (output, benchmark dir, sources in it, common sources, npbparams.h)
+    ("npb_is_S", "IS", ["is.c"], ["c_print_results.c", "c_timers.c"],
+     "#define CLASS 'S'\n#define NUM_PROCS 4\n"),
+    # class S parameters of NPB's setparams.c (write_dt_info)
+    ("npb_dt_S", "DT", ["dt.c", "DGraph.c"], ["c_print_results.c", "c_timers.c", "randdp.c"],
+     "#define CLASS 'S'\n#define NUM_PROCS 5\n#define NUM_SAMPLES 1728\n"
+     "#define STD_DEVIATION 128\n#define NUM_SOURCES 4\n"),
+]
+
+
+def build_npb(verbose=False):
+    if not os.path.isdir(NPB):
+        return []
+    sys.path.insert(0, HERE)
+    import gpumpicc
+    out_dir = os.path.join(ROOT, "tests", "programs", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for name, bdir, srcs, common, params in NPB_PROGRAMS:
+        tmp = tempfile.mkdtemp(prefix="gpumpi_npb_")
+        try:
+            proj = os.path.join(tmp, "npb")
+            os.makedirs(os.path.join(proj, bdir))
+            os.makedirs(os.path.join(proj, "common"))
+            for f in srcs + (["DGraph.h"] if bdir == "DT" else []):
+                shutil.copy(os.path.join(NPB, bdir, f), os.path.join(proj, bdir, f))
+            for f in common + ["c_timers.h"]:
+                shutil.copy(os.path.join(NPB, "common", f), os.path.join(proj, "common", f))
+            with open(os.path.join(proj, bdir, "npbparams.h"), "w") as fh:
+                fh.write(params + NPB_COMMON_DEFS)
+            files = [os.path.join(proj, bdir, f) for f in srcs] + \
+                    [os.path.join(proj, "common", f) for f in common]
+            exe = gpumpicc.build(files, os.path.join(tmp, name), project=proj,
+                                 build_dir=os.path.join(tmp, name + ".gpumpi"), ranks=64)
+            final = os.path.join(out_dir, name)
+            shutil.copy(exe, final)
+            if verbose:
+                print(f"built {final}", flush=True)
+            outs.append(final)
+        except Exception as e:  # a converter gap is reported, the rest of the build goes on
+            sys.stderr.write(f"NPB {name}: {e}\n")
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    return outs
+
+
 def build(verbose=False):
     """Compile libgpumpi.so (always recompiles); atomically replaces the old library."""
     fd, tmp = tempfile.mkstemp(prefix="libgpumpi.", suffix=".so.tmp", dir=HERE)
@@ -114,6 +170,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     build_programs(verbose)
     build_c_programs(verbose)
+    build_npb(verbose)
     return LIB
 
 

@@ -900,7 +900,8 @@ static int a2av_configure(gpumpi_a2av_plan* pl, const gpumpi_a2av_tune& t) {
     pl->tune.nontemporal = t.nontemporal > 0 ? 1 : 0;
     pl->tune.chunk_rounds = R;
     pl->tune.defer_straddle = t.defer_straddle > 0 ? 1 : 0;
-    pl->tune.pipeline = t.pipeline != 0 ? 1 : 0;
+    if (t.pipeline < 0 || t.pipeline > 3) return GPUMPI_E_ARG;
+    pl->tune.pipeline = t.pipeline;
     pl->dev.chunk_lines = (uint32_t)(kA2avCopyThreads * U * R);
     a2av_size_grid(pl);
     pl->dev2.chunk_lines = pl->dev.chunk_lines;
@@ -930,10 +931,13 @@ static int a2av_launch(gpumpi_a2av_plan* pl, hipStream_t s, bool timed) {
 }
 
 // The copy of the step whose tables are in workspace `cur`, with the plan of the next
-// step (into the other workspace) in its first P workgroups when `plan_next`.
-static int a2av_launch_copy(gpumpi_a2av_plan* pl, hipStream_t s, int cur, bool plan_next) {
+// step (into the other workspace) in its first P workgroups when `plan_next`; that plan
+// also computes the chunk scan in its last workgroup when `scan_too`.
+static int a2av_launch_copy(gpumpi_a2av_plan* pl, hipStream_t s, int cur, bool plan_next,
+                            bool scan_too = true) {
     A2avDev& a = cur == 0 ? pl->dev : pl->dev2;
-    A2avDev& b = cur == 0 ? pl->dev2 : pl->dev;
+    A2avDev b = cur == 0 ? pl->dev2 : pl->dev;
+    if (!scan_too) b.plan_done = nullptr;
     void* kargs[] = {(void*)&a, (void*)&b};
     const int grid = pl->grid + (plan_next ? a.P : 0);
     if (hipLaunchKernel(a2av_copy_for(pl->tune.lines_per_lane, pl->tune.nontemporal > 0,
@@ -944,19 +948,55 @@ static int a2av_launch_copy(gpumpi_a2av_plan* pl, hipStream_t s, int cur, bool p
     return GPUMPI_OK;
 }
 
-// `steps` steps.  Pipelined: plan and scan of the first step, then one copy kernel per
-// step that also plans the next step into the other workspace (the last one plans
-// nothing).  Otherwise three kernels per step.
+static int a2av_launch_plan_scan(gpumpi_a2av_plan* pl, hipStream_t s, int ws, bool plan) {
+    A2avDev& d = ws == 0 ? pl->dev : pl->dev2;
+    void* kargs1[] = {(void*)&d};
+    if (plan && hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(d.P), dim3(256), kargs1, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    if (hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    return GPUMPI_OK;
+}
+
+// `steps` steps, in the pipeline mode of the tune:
+//   0  plan, scan and copy kernels per step;
+//   1  each copy kernel also plans the next step into the other workspace (its first P
+//      workgroups, no fences) and a scan kernel follows it;
+//   2  as 1, and the plan's last workgroup computes the scan (one kernel per step);
+//   3  plan and scan of step i+1 on a second stream, concurrent with copy i (graph
+//      branches; copy i+1 waits for them, plan i+2 for copy i, which used its workspace).
 static int a2av_launch_steps(gpumpi_a2av_plan* pl, hipStream_t s, int steps) {
     int rc = GPUMPI_OK;
-    if (pl->tune.pipeline) {
-        A2avDev& d = pl->dev;
-        void* kargs1[] = {(void*)&d};
-        if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(d.P), dim3(256), kargs1, 0, s) != hipSuccess ||
-            hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess)
-            return GPUMPI_E_HIP;
-        for (int i = 0; i < steps && rc == GPUMPI_OK; i++)
-            rc = a2av_launch_copy(pl, s, i & 1, i + 1 < steps);
+    const int mode = pl->tune.pipeline;
+    if (mode == 1 || mode == 2) {
+        rc = a2av_launch_plan_scan(pl, s, 0, true);
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
+            const bool next = i + 1 < steps;
+            rc = a2av_launch_copy(pl, s, i & 1, next, mode == 2);
+            if (rc == GPUMPI_OK && next && mode == 1) rc = a2av_launch_plan_scan(pl, s, (i + 1) & 1, false);
+        }
+    } else if (mode == 3) {
+        hipStream_t sp;
+        if (hipStreamCreateWithFlags(&sp, hipStreamNonBlocking) != hipSuccess) return GPUMPI_E_HIP;
+        std::vector<hipEvent_t> ev((size_t)2 * steps + 2);
+        for (hipEvent_t& e : ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        hipEvent_t* planned = ev.data();          // [steps]: plan + scan of step i done
+        hipEvent_t* copied = ev.data() + steps;   // [steps]: copy of step i done
+        hipEvent_t fork = ev[(size_t)2 * steps], join = ev.back();
+        hipEventRecord(fork, s);
+        hipStreamWaitEvent(sp, fork, 0);
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
+            if (i >= 2) hipStreamWaitEvent(sp, copied[i - 2], 0);
+            rc = a2av_launch_plan_scan(pl, sp, i & 1, true);
+            hipEventRecord(planned[i], sp);
+            hipStreamWaitEvent(s, planned[i], 0);
+            if (rc == GPUMPI_OK) rc = a2av_launch_copy(pl, s, i & 1, false);
+            hipEventRecord(copied[i], s);
+        }
+        hipEventRecord(join, sp);  // join: the last plan (already before the last copy)
+        hipStreamWaitEvent(s, join, 0);
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+        hipStreamDestroy(sp);
     } else {
         for (int i = 0; i < steps && rc == GPUMPI_OK; i++) rc = a2av_launch(pl, s, false);
     }
@@ -1026,7 +1066,7 @@ int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs
         x.plan_done = (uint32_t*)(b + ws - 128);
     }
     for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
-    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4, 0, 1};
+    gpumpi_a2av_tune t{kA2avCopyThreads, 0, 4, 1, 4, 0, 2};
     if (tune) {
         if (tune->block_threads > 0) t.block_threads = tune->block_threads;
         if (tune->blocks_per_cu >= 0) t.blocks_per_cu = tune->blocks_per_cu;
@@ -1113,8 +1153,10 @@ int gpumpi_a2av_pipeline_times(gpumpi_a2av_plan* pl, int32_t steps, void* stream
         hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess)
         rc = GPUMPI_E_HIP;
     hipEventRecord(ev[1], s);
+    const bool scan_in_plan = pl->tune.pipeline != 1;
     for (int i = 0; i < steps && rc == GPUMPI_OK; i++) {
-        rc = a2av_launch_copy(pl, s, i & 1, i + 1 < steps);
+        rc = a2av_launch_copy(pl, s, i & 1, i + 1 < steps, scan_in_plan);
+        if (rc == GPUMPI_OK && !scan_in_plan && i + 1 < steps) rc = a2av_launch_plan_scan(pl, s, (i + 1) & 1, false);
         hipEventRecord(ev[(size_t)i + 2], s);
     }
     if (rc == GPUMPI_OK && hipEventSynchronize(ev.back()) == hipSuccess) {
@@ -1209,11 +1251,39 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
                     best = pl->tune;
                 }
             }
+    // then the pipeline mode, on the graph replays it applies to: 10-step graphs, the
+    // best of three replays per mode
+    int rc = a2av_configure(pl, best);
+    double best_step = 1e30;
+    int best_mode = best.pipeline;
+    for (int mode = 0; mode <= 3 && rc == GPUMPI_OK; mode++) {
+        gpumpi_a2av_tune t = best;
+        t.pipeline = mode;
+        a2av_drop_graph(pl);
+        if (a2av_configure(pl, t) != GPUMPI_OK) continue;
+        if (gpumpi_a2av_graph_replay(pl, 10, s) != GPUMPI_OK) break;  // capture + warm-up
+        double mode_best = 1e30;
+        for (int r = 0; r < 3; r++) {
+            hipEventRecord(e0, s);
+            gpumpi_a2av_graph_replay(pl, 10, s);
+            hipEventRecord(e1, s);
+            hipEventSynchronize(e1);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            mode_best = std::min(mode_best, 1000.0 * ms / 10);
+        }
+        if (mode_best < best_step) {
+            best_step = mode_best;
+            best_mode = mode;
+        }
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    const int rc = a2av_configure(pl, best);
+    a2av_drop_graph(pl);
+    best.pipeline = best_mode;
+    rc = a2av_configure(pl, best);
     if (chosen) *chosen = pl->tune;
-    if (best_us) *best_us = best_t;
+    if (best_us) *best_us = best_step < 1e30 ? best_step : best_t;
     return rc;
 }
 
@@ -1740,7 +1810,11 @@ int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_
         return GPUMPI_E_ARG;
     *out = nullptr;
     const int es = type_size(datatype);
-    const void* k = es > 0 ? batched_reduce_kernel_for(datatype, op) : nullptr;
+    int32_t cus = 0;
+    if (gpumpi_device_info(&cus, nullptr, nullptr) != GPUMPI_OK) return GPUMPI_E_HIP;
+    const uint64_t bytes = es > 0 ? (uint64_t)count * (uint64_t)es : 0;
+    const bool wide = bytes / (1024ull * kReduceWideU) >= 4ull * (uint64_t)cus;
+    const void* k = es > 0 ? batched_reduce_kernel_for(datatype, op, wide) : nullptr;
     if (!k) {
         g_host_error = "batched reduction: the pair (datatype " + std::to_string(datatype) +
                        ", operator " + std::to_string(op) + ") is not on the batched path "
@@ -1748,11 +1822,8 @@ int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_
                        "MPI_Reduce on the rank grid takes every legal pair";
         return GPUMPI_E_ARG;
     }
-    int32_t cus = 0;
-    if (gpumpi_device_info(&cus, nullptr, nullptr) != GPUMPI_OK) return GPUMPI_E_HIP;
     gpumpi_reduce_plan* pl = new gpumpi_reduce_plan();
     pl->kernel = k;
-    const uint64_t bytes = (uint64_t)count * (uint64_t)es;
     const uint64_t written = bytes * (root < 0 ? (uint64_t)P : 1u);
     pl->dev.P = P;
     pl->dev.root = root;
@@ -1760,10 +1831,8 @@ int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_
     pl->dev.nt = written >= (64ull << 20) ? 1 : 0;  // outputs larger than the L2s stream past them
     pl->dev.sendbufs = (const char*)sendbufs;
     pl->dev.recvbufs = (const char*)recvbufs;
-    const uint64_t nvec = bytes / 16;
-    uint64_t g = (nvec + kReduceThreads - 1) / kReduceThreads;
-    if (g < 1) g = 1;
-    if (g > (1u << 20)) g = 1u << 20;
+    uint64_t g = batched_reduce_grid(bytes, wide ? kReduceWideU : kReduceNarrowU);
+    if (g > (1u << 20)) g = 1u << 20;  // the waves stride over the rest
     pl->grid = (int)g;
     *out = pl;
     return GPUMPI_OK;

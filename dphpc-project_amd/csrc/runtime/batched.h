@@ -100,6 +100,9 @@ __device__ __forceinline__ int64_t wave_search_le(const uint64_t* a, int64_t n, 
 // kept in registers: the quarters' count totals meet in LDS to give each quarter's
 // starting displacement, then each wave scans and writes.  Sources past 256 per quarter
 // are summed in a first pass and reloaded in the second.
+// COHERENT: the row's byte count is stored with an agent-scope atomic store (written
+// through to memory), for a reader in the same kernel on another XCD.
+template <bool COHERENT = false>
 __device__ __forceinline__ void a2av_plan_row(const A2avDev& p, int64_t d, int64_t* s_tot) {
     const int P = p.P;
     const int l = lane();
@@ -157,7 +160,11 @@ __device__ __forceinline__ void a2av_plan_row(const A2avDev& p, int64_t d, int64
             p.seg_src[o] = (uint64_t)(uintptr_t)(sbq + sdq * p.elem);
         }
     }
-    if (threadIdx.x == 0) p.row_bytes[d] = (uint64_t)row * (uint64_t)p.elem;
+    if (threadIdx.x == 0) {
+        const uint64_t rb = (uint64_t)row * (uint64_t)p.elem;
+        if constexpr (COHERENT) st_relaxed(&p.row_bytes[d], rb);
+        else p.row_bytes[d] = rb;
+    }
 }
 
 __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
@@ -166,8 +173,9 @@ __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
 }
 
 // ------------------------------------------------------------------------------ scan
-// Run by one workgroup of NT threads (NT a multiple of 64, at most 1024).
-template <int NT>
+// Run by one workgroup of NT threads (NT a multiple of 64, at most 1024).  COHERENT: the
+// row byte counts are read with agent-scope atomic loads (written in the same kernel).
+template <int NT, bool COHERENT = false>
 __device__ __forceinline__ void a2av_scan_rows(const A2avDev& p, uint64_t* part, int64_t* bpart) {
     constexpr int NW = NT / kWave;
     const int P = p.P;
@@ -180,7 +188,7 @@ __device__ __forceinline__ void a2av_scan_rows(const A2avDev& p, uint64_t* part,
         uint64_t ch = 0;
         int64_t nb = 0;
         if (d < P) {
-            nb = (int64_t)p.row_bytes[d];
+            nb = COHERENT ? (int64_t)ld_relaxed(&p.row_bytes[d]) : (int64_t)p.row_bytes[d];
             const uint64_t rb = (uint64_t)(uintptr_t)p.recvbufs[d];
             const uint64_t lines = nb ? ((rb + (uint64_t)nb + 15) >> 4) - ((rb >> 7) << 3) : 0;
             ch = (lines + C - 1) / C;
@@ -214,28 +222,36 @@ __global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
 
 // Plan of the NEXT step run by the first P workgroups of a copy kernel (the step's only
 // global dependency -- the receive displacements -- is then resolved while the current
-// step streams, instead of in two kernels of its own on the critical path).  The
-// workgroup that finishes last computes the chunk prefix: every workgroup releases its
-// tables at agent scope before it counts itself done, and the last one acquires them.
-// The tables live in the other of two workspaces, so the copy running beside them is
-// not disturbed.
+// step streams, instead of in kernels of its own on the critical path).  The tables go to
+// the other of two workspaces, so the copy running beside them is not disturbed.
+//
+// With pn.plan_done set, the workgroup that finishes last also computes the chunk prefix.
+// Only the row byte counts cross workgroups inside the kernel (everything else is read by
+// the next kernel): each planner stores its count with an agent-scope atomic store, which
+// writes through to memory, waits for that store to complete, then counts itself done;
+// the last one reads the counts with agent-scope atomic loads, which miss the L2.  No L2
+// write-back or invalidate is needed (agent-scope fences in 512 workgroups, writing back
+// their XCDs' L2s under the streaming copy, cost 35-60 us per step).
+// With pn.plan_done == nullptr the workgroups only plan; a scan kernel follows.
 __device__ void a2av_plan_fused(const A2avDev& pn, int64_t d) {
     __shared__ int64_t s_tot[4];
     __shared__ uint64_t s_part[4];
     __shared__ int64_t s_bpart[4];
     __shared__ int32_t s_last;
-    a2av_plan_row(pn, d, s_tot);
-    __threadfence();
-    __syncthreads();
+    if (!pn.plan_done) {  // block-uniform
+        a2av_plan_row<false>(pn, d, s_tot);
+        return;
+    }
+    a2av_plan_row<true>(pn, d, s_tot);
     if (threadIdx.x == 0) {
+        wait_mem_done();
         const uint32_t old = __hip_atomic_fetch_add(pn.plan_done, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         s_last = old == (uint32_t)pn.P - 1u;
     }
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
-    a2av_scan_rows<kA2avCopyThreads>(pn, s_part, s_bpart);
+    a2av_scan_rows<kA2avCopyThreads, true>(pn, s_part, s_bpart);
     if (threadIdx.x == 0) *pn.plan_done = 0u;  // next used two kernels later
 }
 

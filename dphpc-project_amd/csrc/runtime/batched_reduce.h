@@ -31,34 +31,60 @@ struct ReduceDev {
     const char* recvbufs;    // [P] device pointers
 };
 
-template <class T, int OP>
+// Launch shapes: each wave folds U KiB of every rank buffer per visit (U 16-byte columns
+// per lane, 1 KiB apart) with D ranks' loads in flight, U * D columns in all.  Measured on
+// MI355X, 1024 ranks (2 x P x N x s bytes per step): at 1 MiB per rank U = 2, D = 8 reached
+// 66 % of HBM peak and U = 1 only 40-42 % at any depth (a wave visits every rank buffer
+// once per KiB, and the visits, not the bytes in flight, bound it); at 8 MiB per rank
+// U = 8, D = 4 reached 73 %, but at 1 MiB it leaves too few waves (128 on 256 CUs).
+// The plan takes the wide shape when a rank buffer gives at least four of its tiles per CU.
+constexpr int kReduceNarrowU = 2, kReduceNarrowD = 8;
+constexpr int kReduceWideU = 8, kReduceWideD = 4;
+
+template <class T, int OP, int U, int D>
 __global__ void __launch_bounds__(kReduceThreads) batched_reduce_kernel(ReduceDev p) {
     const uint64_t nvec = (uint64_t)p.count * sizeof(T) / 16;
     const PtrTable src{p.sendbufs, 8}, dst{p.recvbufs, 8};
     const uint64_t one = p.root >= 0 ? *(const uint64_t*)(p.recvbufs + 8 * (int64_t)p.root) : 0;
-    const uint64_t stride = (uint64_t)gridDim.x * kReduceThreads;
-    for (uint64_t v0 = (uint64_t)blockIdx.x * kReduceThreads + (threadIdx.x & ~(kWave - 1));
-         v0 < nvec; v0 += stride) {  // wave-uniform
-        const uint64_t v = v0 + lane();
-        const bool valid = v < nvec;
-        const u32x4 acc = fold_column<T, OP>(src, p.P, 16 * v, valid);
-        if (p.nt) fold_store<true>(dst, p.P, one, 16 * v, valid, acc);
-        else fold_store<false>(dst, p.P, one, 16 * v, valid, acc);
+    constexpr uint64_t kTile = (uint64_t)kWave * U;  // 16-byte columns per wave and visit
+    const uint64_t ntiles = (nvec + kTile - 1) / kTile;
+    const uint64_t wpb = kReduceThreads / kWave;
+    const uint64_t nw = (uint64_t)gridDim.x * wpb;
+    for (uint64_t t = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6); t < ntiles; t += nw) {
+        const uint64_t v0 = t * kTile + (uint64_t)lane();
+        uint32_t valid = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (v0 + (uint64_t)kWave * u < nvec) valid |= 1u << u;
+        u32x4 acc[U];
+        fold_columns<T, OP, U, D>(src, p.P, 16 * v0, valid, acc);
+        if (p.nt) fold_store_cols<true, U>(dst, p.P, one, 16 * v0, valid, acc);
+        else fold_store_cols<false, U>(dst, p.P, one, 16 * v0, valid, acc);
     }
     // elements past the last whole 16-byte column
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x < kWave)
         fold_scalar_tail<T, OP>(src, p.P, dst, p.P, one, (int64_t)(nvec * 16 / sizeof(T)), p.count);
 }
 
-// Kernel for a (datatype, built-in operator) pair of the batched path, or nullptr.
-__host__ inline const void* batched_reduce_kernel_for(int dt, int op) {
-#define GPUMPI_RED_OPS(T)                                                            \
-    switch (op) {                                                                    \
-    case kOpMax: return (const void*)&batched_reduce_kernel<T, kOpMax>;              \
-    case kOpMin: return (const void*)&batched_reduce_kernel<T, kOpMin>;              \
-    case kOpSum: return (const void*)&batched_reduce_kernel<T, kOpSum>;              \
-    case kOpProd: return (const void*)&batched_reduce_kernel<T, kOpProd>;            \
-    default: return nullptr;                                                         \
+// Workgroups for `bytes` per rank: one wave per U KiB tile.
+__host__ inline uint64_t batched_reduce_grid(uint64_t bytes, int U) {
+    const uint64_t tiles = (bytes / 16 + (uint64_t)kWave * U - 1) / ((uint64_t)kWave * U);
+    const uint64_t wpb = kReduceThreads / kWave;
+    const uint64_t g = (tiles + wpb - 1) / wpb;
+    return g < 1 ? 1 : g;
+}
+
+// Kernel for a (datatype, built-in operator) pair of the batched path and a shape (0 narrow,
+// 1 wide), or nullptr.
+template <int U, int D>
+__host__ inline const void* batched_reduce_kernel_shape(int dt, int op) {
+#define GPUMPI_RED_OPS(T)                                                        \
+    switch (op) {                                                                \
+    case kOpMax: return (const void*)&batched_reduce_kernel<T, kOpMax, U, D>;    \
+    case kOpMin: return (const void*)&batched_reduce_kernel<T, kOpMin, U, D>;    \
+    case kOpSum: return (const void*)&batched_reduce_kernel<T, kOpSum, U, D>;    \
+    case kOpProd: return (const void*)&batched_reduce_kernel<T, kOpProd, U, D>;  \
+    default: return nullptr;                                                     \
     }
     switch (dt) {
     case MPI_INT: case MPI_INT32_T: GPUMPI_RED_OPS(int32_t)
@@ -70,6 +96,11 @@ __host__ inline const void* batched_reduce_kernel_for(int dt, int op) {
     default: return nullptr;
     }
 #undef GPUMPI_RED_OPS
+}
+
+__host__ inline const void* batched_reduce_kernel_for(int dt, int op, bool wide) {
+    return wide ? batched_reduce_kernel_shape<kReduceWideU, kReduceWideD>(dt, op)
+                : batched_reduce_kernel_shape<kReduceNarrowU, kReduceNarrowD>(dt, op);
 }
 
 }  // namespace gpumpi

@@ -43,20 +43,28 @@ __device__ __forceinline__ u32x4 fold_vec_op(u32x4 x, u32x4 acc) {
     return out;
 }
 
-// Folds the 16-byte column at byte offset `off` of every source (ranks 0..P-1 in order)
-// and returns it; `valid` lanes only load.  Wave-uniform P; all lanes must call.
-template <class T, int OP>
-__device__ __forceinline__ u32x4 fold_column(const PtrTable& src, int P, uint64_t off, bool valid) {
-    constexpr int D = kFoldDepth;
-    u32x4 ring[D];
+// Folds U 16-byte columns per lane -- at byte offsets off + 1024 * u, u < U, of every
+// source (ranks 0..P-1 in order) -- with D ranks' loads in flight, into acc[u].  Lanes
+// load column u only where bit u of `valid` is set.  A wave thus reads U KiB of a rank's
+// buffer per visit.  Wave-uniform P; all lanes must call.
+template <class T, int OP, int U, int D>
+__device__ __forceinline__ void fold_columns(const PtrTable& src, int P, uint64_t off, uint32_t valid,
+                                             u32x4 (&acc)[U]) {
+    static_assert(D >= 1 && D <= kWave && kWave % D == 0, "D must divide the wave size");
+    u32x4 ring[D][U];
     uint64_t pc = src.lane_chunk(0, P);
     uint64_t pn = P > kWave ? src.lane_chunk(kWave, P) : 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        ring[k] = u32x4{0u, 0u, 0u, 0u};
-        if (k < P && valid) ring[k] = gload16(readlane64(pc, k) + off);
+        const uint64_t b = k < P ? readlane64(pc, k) + off : 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ring[k][u] = u32x4{0u, 0u, 0u, 0u};
+            if (k < P && (valid >> u & 1u)) ring[k][u] = gload16(b + 1024u * u);
+        }
     }
-    u32x4 acc = u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = u32x4{0u, 0u, 0u, 0u};
     for (int r0 = 0; r0 < P; r0 += kWave) {
         for (int j0 = 0; j0 < kWave; j0 += D) {
             if (r0 + j0 >= P) break;
@@ -64,20 +72,33 @@ __device__ __forceinline__ u32x4 fold_column(const PtrTable& src, int P, uint64_
             for (int k = 0; k < D; k++) {
                 const int r = r0 + j0 + k;
                 if (r < P) {
-                    const u32x4 x = ring[k];
-                    const int jn = j0 + k + D;  // chunk-relative index of rank r + D
-                    if (r + D < P && valid) {
-                        const uint64_t p = jn < kWave ? readlane64(pc, jn) : readlane64(pn, jn - kWave);
-                        ring[k] = gload16(p + off);
+                    u32x4 x[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) x[u] = ring[k][u];
+                    if (r + D < P) {
+                        const int jn = j0 + k + D;  // chunk-relative index of rank r + D
+                        const uint64_t b =
+                            (jn < kWave ? readlane64(pc, jn) : readlane64(pn, jn - kWave)) + off;
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                            if (valid >> u & 1u) ring[k][u] = gload16(b + 1024u * u);
                     }
-                    acc = r == 0 ? x : fold_vec_op<T, OP>(x, acc);
+#pragma unroll
+                    for (int u = 0; u < U; u++) acc[u] = r == 0 ? x[u] : fold_vec_op<T, OP>(x[u], acc[u]);
                 }
             }
         }
         pc = pn;
         pn = r0 + 2 * kWave < P ? src.lane_chunk(r0 + 2 * kWave, P) : 0;
     }
-    return acc;
+}
+
+// One column per lane (the rank grid's slices), kFoldDepth ranks in flight.
+template <class T, int OP>
+__device__ __forceinline__ u32x4 fold_column(const PtrTable& src, int P, uint64_t off, bool valid) {
+    u32x4 acc[1];
+    fold_columns<T, OP, 1, kFoldDepth>(src, P, off, valid ? 1u : 0u, acc);
+    return acc[0];
 }
 
 // Stores a lane's folded column to every destination of the table (ndst of them), or to
@@ -94,6 +115,29 @@ __device__ __forceinline__ void fold_store(const PtrTable& dst, int ndst, uint64
         const int n = ndst - d0 < kWave ? ndst - d0 : kWave;
         for (int j = 0; j < n; j++)
             if (valid) gstore16<NT>(readlane64(pd, j) + off, v);
+    }
+}
+
+// Stores a lane's U folded columns (offsets off + 1024 * u where bit u of `valid` is set)
+// to every destination of the table, or to the one address `one` when non-zero.
+template <bool NT, int U>
+__device__ __forceinline__ void fold_store_cols(const PtrTable& dst, int ndst, uint64_t one,
+                                                uint64_t off, uint32_t valid, const u32x4 (&v)[U]) {
+    if (one) {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (valid >> u & 1u) gstore16<NT>(one + off + 1024u * u, v[u]);
+        return;
+    }
+    for (int d0 = 0; d0 < ndst; d0 += kWave) {
+        const uint64_t pd = dst.lane_chunk(d0, ndst);
+        const int n = ndst - d0 < kWave ? ndst - d0 : kWave;
+        for (int j = 0; j < n; j++) {
+            const uint64_t b = readlane64(pd, j) + off;
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (valid >> u & 1u) gstore16<NT>(b + 1024u * u, v[u]);
+        }
     }
 }
 

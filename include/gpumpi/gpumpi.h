@@ -137,10 +137,14 @@ typedef struct gpumpi_a2av_tune {
     int32_t defer_straddle;     /* 1 = lines holding several segments are assembled by one
                                    thread per segment in leading copy blocks (pays for short
                                    segments), 0 = by the copy loop, -1 = default (0) */
-    int32_t pipeline;           /* graph replays: 1 = each step's copy kernel also plans the
-                                   next step (into a second workspace), so plan and scan
-                                   leave the critical path; 0 = plan, scan, copy kernels per
-                                   step; -1 = default (1) */
+    int32_t pipeline;           /* graph replays (two workspaces, the next step planned
+                                   while the current one copies): 0 = plan, scan, copy
+                                   kernels per step; 1 = each copy kernel also plans the next
+                                   step in its first P workgroups, a scan kernel follows;
+                                   2 = as 1 with the scan in the plan's last workgroup
+                                   (one kernel per step);
+                                   3 = plan and scan on a concurrent graph branch;
+                                   -1 = default (2) */
 } gpumpi_a2av_tune;
 
 int gpumpi_a2av_plan_create(int32_t P, int32_t elem_bytes, void* const* sendbufs,
@@ -153,12 +157,15 @@ void gpumpi_a2av_plan_destroy(gpumpi_a2av_plan* plan);
 int gpumpi_a2av_execute(gpumpi_a2av_plan* plan, void* stream);
 /* Capture `steps_per_graph` steps into a hipGraph once, then replay it. */
 int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* plan, int32_t steps_per_graph, void* stream);
-/* Sweep launch shapes on the live buffers, keep the fastest; returns the chosen shape. */
+/* Sweep launch shapes on the live buffers (single steps), then the pipeline modes (10-step
+ * graph replays), keep the fastest; returns the chosen shape and best_us = its time per
+ * step in graph replays. */
 int gpumpi_a2av_autotune(gpumpi_a2av_plan* plan, void* stream, gpumpi_a2av_tune* chosen,
                          double* best_us);
-/* Kernel durations of `steps` (>= 2) pipelined steps launched directly, timed with HIP
- * events on `stream`: us[0] = the first step's plan + scan, us[1] = the average copy kernel
- * (each but the last also planning the next step), us[2] = the sequence's time per step. */
+/* Kernel durations of `steps` (>= 2) steps of pipeline mode 1 or 2 launched directly, timed
+ * with HIP events on `stream`: us[0] = the first step's plan + scan, us[1] = the average
+ * copy kernel (each but the last also planning the next step; in mode 1 with the scan
+ * kernel after it), us[2] = the sequence's time per step. */
 int gpumpi_a2av_pipeline_times(gpumpi_a2av_plan* plan, int32_t steps, void* stream, double* us);
 /* Per-kernel durations of a step (profiling): enable = 1 makes gpumpi_a2av_execute record
  * HIP events around its kernels, 0 stops it, -1 reads us[3] = {plan, scan, copy} in

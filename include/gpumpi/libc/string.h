@@ -1,9 +1,10 @@
 // Device string.h for converted programs (gpu_libs/gpu_libc/string.cuh): memcpy and
 // memset are the HIP device builtins; memcmp, memchr, memmove and the str* functions are
-// written here.
+// written here (strdup / strndup allocate from the runtime's device heap).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string.h>
+#include "../gpu_malloc.h"
 
 namespace gpumpi_libc {
 __device__ inline size_t str_len(const char* s) {
@@ -82,9 +83,21 @@ __device__ inline void* mem_move(void* d, const void* s, size_t n) {
     }
     return d;
 }
+__device__ inline char* str_ndup(const char* s, size_t n) {
+    size_t m = 0;
+    while (m < n && s[m]) m++;
+    char* d = (char*)__gpu_malloc(m + 1);
+    if (!d) return nullptr;
+    for (size_t i = 0; i < m; i++) d[i] = s[i];
+    d[m] = 0;
+    return d;
+}
+__device__ inline char* str_dup(const char* s) { return str_ndup(s, (size_t)-1); }
 }  // namespace gpumpi_libc
 
 #define GPUMPI_LIBC_STRING 1
+#define strdup(s) ::gpumpi_libc::str_dup(s)
+#define strndup(s, n) ::gpumpi_libc::str_ndup((s), (n))
 #define memcmp(a, b, n) ::gpumpi_libc::mem_cmp((a), (b), (n))
 #define memchr(s, c, n) ::gpumpi_libc::mem_chr((s), (c), (n))
 #define memmove(d, s, n) ::gpumpi_libc::mem_move((d), (s), (n))

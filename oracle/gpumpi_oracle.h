@@ -14,19 +14,29 @@
  *   orc_fill_*          Seeded input generators for the five workloads.  They are
  *                       definitions (the device generators must match them bit
  *                       for bit); pinned through the Philox KATs plus the
- *                       closed-form ranges/moments checked in the tests.
+ *                       closed-form ranges/moments checked in the tests
+ *                       (tests/test_oracle.py).  orc_fill_*_at draw the same
+ *                       streams at given indices, equal to the full streams there.
  *   orc_mcpi_hits       Monte-Carlo pi hit count per rank.  Pinned by brute
  *                       force on tiny n and by the pi estimate's error bound.
  *   orc_alltoallv       MPI_Alltoallv semantics, gpu_mpi/mpi.cu:363-408 (counts
- *                       and displacements in elements).  Pinned by the
- *                       permutation invariant and by host MPICH
- *                       (oracle/mpi/a2av_check.c).
+ *                       and displacements in elements).  Pinned by an independent
+ *                       Python block permutation (tests/test_oracle.py) and by host
+ *                       MPICH's MPI_Alltoallv byte for byte (oracle/mpi/a2av_check.c,
+ *                       tests/test_oracle_pins.py).
+ *   orc_counts_transpose  The Alltoall of the send counts and the packed receive
+ *                       displacements (is.c:758-781).  Pinned by a transpose and an
+ *                       exclusive prefix sum (tests/test_oracle_pins.py).
  *   orc_reduce          Reference fold order of MPI_Reduce, gpu_mpi/mpi.cu:209-225
  *                       (recv = buf_0; recv = op(buf_r, recv) for r = 1..P-1),
  *                       operator bodies gpu_mpi/operators.cu:73-151 and the
  *                       type/op legality table operators.cu:153-200.  Pinned by
  *                       the reference test known answers (tests/collectives_test.cu
- *                       :6-98) and, for FP64, by host MPICH within 1e-12 normwise.
+ *                       :6-98), a cancellation case that fixes the left-to-right
+ *                       fold, numpy for integer and bitwise operators, and host
+ *                       MPICH's MPI_Reduce/MPI_Allreduce: integers and FP64 MAX
+ *                       equal, FP64 SUM within 1e-12 * sum |x| per element
+ *                       (oracle/mpi/reduce_check.c, tests/test_oracle_pins.py).
  *   orc_comm_split      MPI_Comm_split ordering by (key, old rank),
  *                       gpu_mpi/communicator.cu:154-213.  Pinned by the
  *                       reference table tests/communicator_test.cu:93-152.
@@ -36,10 +46,15 @@
  *                       create_seq 520-537, rank() 631-928, full_verify
  *                       545-621.  Pinned by NPB's own partial-verification
  *                       known answers (is.c:264-287, adjustments 849-905):
- *                       passed == 5*MAX_ITERATIONS + P for classes S, W, A.
+ *                       passed == 5*MAX_ITERATIONS + P for classes S, W, A, and
+ *                       orc_npb_is_iteration passing all five test keys in each of
+ *                       iterations 1..10 run in order on one key array.
  *   orc_bucket_*        The NPB-style bucket exchange applied to the workload's
  *                       uniform / Zipf keys (same rank() steps).  Pinned by the
  *                       global order equalling a plain sort of all keys.
+ *   orc_local_sort      The counting rank and placement of rank() / full_verify
+ *                       (is.c:807-833, 545-560) as an ascending sort.  Pinned by
+ *                       numpy.sort (tests/test_oracle_pins.py).
  *   orc_particles_*     Particle exchange on a periodic gx*gy*gz rank grid: every
  *                       rank packs its migrants by destination rank and the step
  *                       is one MPI_Alltoallv (orc_alltoallv above), stayers first.

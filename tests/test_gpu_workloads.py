@@ -73,6 +73,30 @@ def test_p2p_payloads(mode, nbytes, pkg, oracle, torch_gpu):
         assert np.array_equal(recv[r], fill_bytes(oracle, seed, origin, nbytes)), r
 
 
+@pytest.mark.parametrize("mode", [0, 3])
+@pytest.mark.parametrize("nbytes", [8, 4 << 20, 64 << 20])
+def test_p2p_256_ranks(mode, nbytes, pkg, oracle, torch_gpu):
+    """Config 2 at its own shape: 256 ranks (128 ping-pong pairs, or the ring), up to 64 MiB
+    per message.  Every rank checks every received byte against the Philox generator on
+    the device; the host compares sampled ranks' buffers with the oracle's bytes."""
+    from dphpc_project_amd import programs as pr
+    torch = torch_gpu
+    P, seed = 256, 0x2561
+    print(f"[gpu] p2p P={P} mode={mode} bytes={nbytes}", flush=True)
+    sb = torch.zeros(P * nbytes, dtype=torch.uint8, device="cuda")
+    rb = torch.zeros(P * nbytes, dtype=torch.uint8, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    mis = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    with pkg.Runtime(P, ranks_per_block=4, heap_bytes=256 << 20, timeout_s=60.0) as rt:
+        rt.run("p2p", pr.P2pArgs(seed, mode, 2, 1, 0, nbytes, sb.data_ptr(), rb.data_ptr(),
+                                 sec.data_ptr(), mis.data_ptr()))
+    assert mis.cpu().tolist() == [0] * P
+    for r in (0, 1, 128, 255):
+        origin = (r if r % 2 == 0 else r ^ 1) if mode < 2 else (r - 1) % P
+        got = rb[r * nbytes:(r + 1) * nbytes].cpu().numpy()
+        assert np.array_equal(got, fill_bytes(oracle, seed, origin, nbytes)), r
+
+
 def run_coll(pkg, torch, P, kind, dtype, op, count, root=0, seed=0x1234):
     from dphpc_project_amd import programs as pr
     es = 8 if dtype == MPI_DOUBLE else 4
