@@ -101,6 +101,7 @@ _SIGNATURES = {
     "gpumpi_a2av_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(A2avTune), ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_a2av_pipeline_times": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_a2av_execute_noalloc": (ctypes.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P]),
     "gpumpi_reduce_plan_create": (ctypes.c_int, [_I32, ctypes.c_int64, _I32, _I32, _I32, _P, _P,
                                                  ctypes.POINTER(_P)]),
     "gpumpi_reduce_plan_destroy": (None, [_P]),

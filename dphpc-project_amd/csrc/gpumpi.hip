@@ -1787,6 +1787,93 @@ int64_t gpumpi_job_heap_bytes(const gpumpi_job* job) {
 
 }  // extern "C"
 
+// ==================================================================== allocation-free step
+// One Alltoallv step (plan, scan and copy kernels, as gpumpi_a2av_execute runs them) whose
+// tables -- the per-segment ends and sources, row sizes, chunk prefix and payload total of
+// a plan's workspace -- live in `scratch`, a file-scope device array, so the call allocates
+// nothing.  Every field is written by the step's own plan and scan kernels before the copy
+// kernel reads it; nothing has to start at zero.
+constexpr int32_t kNoAllocMaxRanks = 512;
+constexpr size_t kNoAllocScratchBytes =
+    sizeof(uint64_t) * (2ull * kNoAllocMaxRanks * kNoAllocMaxRanks + 2ull * kNoAllocMaxRanks + 2) + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // scratch's address on each device (host-side view)
+int32_t g_scratch_cus[kMaxDevices];
+}  // namespace
+
+// The device address of `scratch` on the current device, looked up once per device with
+// that device current (the array's name is a host-side stand-in, not a device address).
+static int noalloc_scratch(void** ptr, int32_t* cus) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return GPUMPI_E_HIP;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GPUMPI_E_HIP;
+        int32_t n = 0;
+        if (gpumpi_device_info(&n, nullptr, nullptr) != GPUMPI_OK || n <= 0) return GPUMPI_E_HIP;
+        g_scratch_ptr[dev] = p;
+        g_scratch_cus[dev] = n;
+    }
+    *ptr = g_scratch_ptr[dev];
+    *cus = g_scratch_cus[dev];
+    return GPUMPI_OK;
+}
+
+extern "C" {
+
+int gpumpi_a2av_execute_noalloc(int32_t P, int32_t elem_bytes, void* const* sendbufs,
+                                const int32_t* send_counts, const int32_t* send_displs,
+                                void* const* recvbufs, int32_t* recv_counts,
+                                int32_t* recv_displs, void* stream) {
+    if (P <= 0 || elem_bytes <= 0 || !sendbufs || !send_counts || !send_displs || !recvbufs ||
+        !recv_counts || !recv_displs)
+        return GPUMPI_E_ARG;
+    if (P > kNoAllocMaxRanks) {
+        g_host_error = "gpumpi_a2av_execute_noalloc: " + std::to_string(P) + " ranks exceed the " +
+                       std::to_string(kNoAllocMaxRanks) + " its static workspace holds";
+        return GPUMPI_E_ARG;
+    }
+    void* ws = nullptr;
+    int32_t cus = 0;
+    int rc = noalloc_scratch(&ws, &cus);
+    if (rc != GPUMPI_OK) return rc;
+    A2avDev d{};
+    d.P = P;
+    d.elem = elem_bytes;
+    d.sendbufs = (char* const*)sendbufs;
+    d.scounts = send_counts;
+    d.sdispls = send_displs;
+    d.recvbufs = (char* const*)recvbufs;
+    d.rcounts = recv_counts;
+    d.rdispls = recv_displs;
+    const uint64_t PP = (uint64_t)P * (uint64_t)P;
+    d.seg_end = (uint64_t*)ws;
+    d.seg_src = d.seg_end + PP;
+    d.row_bytes = d.seg_src + PP;
+    d.chunk_off = d.row_bytes + P;
+    d.total_bytes = (int64_t*)(d.chunk_off + P + 1);
+    d.plan_done = nullptr;
+    d.chunk_lines = (uint32_t)(kA2avCopyThreads * 4 * 2);  // 4 lines per lane, 2 rounds
+    d.straddle_blocks = 0;
+    hipStream_t s = (hipStream_t)stream;
+    void* kargs1[] = {(void*)&d};
+    void* kargs2[] = {(void*)&d, (void*)&d};
+    // the copy grid strides over the chunks: 16 workgroups per CU
+    if (hipLaunchKernel((const void*)&a2av_plan_kernel, dim3(P), dim3(256), kargs1, 0, s) != hipSuccess ||
+        hipLaunchKernel((const void*)&a2av_scan_kernel, dim3(1), dim3(1024), kargs1, 0, s) != hipSuccess ||
+        hipLaunchKernel(a2av_copy_for(4, true, false, false), dim3(16 * cus), dim3(kA2avCopyThreads),
+                        kargs2, 0, s) != hipSuccess)
+        return GPUMPI_E_HIP;
+    return GPUMPI_OK;
+}
+
+}  // extern "C"
+
 // ==================================================================== batched Reduce / Allreduce
 struct gpumpi_reduce_plan {
     ReduceDev dev{};

@@ -157,6 +157,18 @@ void gpumpi_a2av_plan_destroy(gpumpi_a2av_plan* plan);
 int gpumpi_a2av_execute(gpumpi_a2av_plan* plan, void* stream);
 /* Capture `steps_per_graph` steps into a hipGraph once, then replay it. */
 int gpumpi_a2av_graph_replay(gpumpi_a2av_plan* plan, int32_t steps_per_graph, void* stream);
+/* One step like gpumpi_a2av_plan_create + gpumpi_a2av_execute + gpumpi_a2av_plan_destroy,
+ * allocating nothing: no hipMalloc or hipFree, and no call that allocates.  No counterpart
+ * in the reference; for callers that cannot allocate -- code under hipGraph capture, tight
+ * serving loops, harnesses that forbid allocation.  The plan tables live in a static
+ * device array of about 4 MiB per GPU, held until the process ends, so P is at most 512
+ * (GPUMPI_E_ARG beyond it).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (concurrent streams, threads) race and give undefined results.  Arguments
+ * as for gpumpi_a2av_plan_create; asynchronous on `stream`. */
+int gpumpi_a2av_execute_noalloc(int32_t P, int32_t elem_bytes, void* const* sendbufs,
+                                const int32_t* send_counts, const int32_t* send_displs,
+                                void* const* recvbufs, int32_t* recv_counts,
+                                int32_t* recv_displs, void* stream);
 /* Sweep launch shapes on the live buffers (single steps), then the pipeline modes (10-step
  * graph replays), keep the fastest; returns the chosen shape and best_us = its time per
  * step in graph replays. */
