@@ -154,13 +154,27 @@ def run_steps(plan, steps, stream):
         plan.graph_replay(rem, stream)
 
 
+# Untimed steps run before the timed ones until the GPU is at its steady-state clocks:
+# at least `warmup` steps and WARMUP_SECONDS of back-to-back steps.  Replays of the same
+# graph speed up over the first few hundred milliseconds of a run (the clocks ramp).
+WARMUP_SECONDS = 0.3
+
+
 def timed_steps(torch, plan, steps, warmup, stream):
-    """Warm-up (at least one replay of every graph the timed steps use, so no capture
-    happens inside the timed region), then ms per step over exactly `steps` steps."""
-    run_steps(plan, max(warmup, GRAPH_STEPS), stream)
+    """Warm-up (every graph the timed steps use replayed at least once, so no capture
+    happens inside the timed region; then steps until the clocks settle), then ms per step
+    over exactly `steps` steps.  Returns (ms per step, warm-up steps run)."""
     if steps % GRAPH_STEPS:
         plan.graph_replay(steps % GRAPH_STEPS, stream)
-    return time_steps(torch, lambda: run_steps(plan, steps, stream), 1) / steps
+    done = steps % GRAPH_STEPS
+    t0 = time.perf_counter()
+    while True:
+        run_steps(plan, 10 * GRAPH_STEPS, stream)
+        torch.cuda.synchronize()
+        done += 10 * GRAPH_STEPS
+        if done >= warmup and time.perf_counter() - t0 >= WARMUP_SECONDS:
+            break
+    return time_steps(torch, lambda: run_steps(plan, steps, stream), 1) / steps, done
 
 
 def kernel_times(torch, plan, stream, reps):
@@ -722,7 +736,7 @@ def main():
     torch.cuda.synchronize()
     ex.verify(torch)
     assert plan.payload_bytes() == ex.payload
-    ms = timed_steps(torch, plan, args.steps, args.warmup, stream)
+    ms, warm_run = timed_steps(torch, plan, args.steps, args.warmup, stream)
     algo = 2 * ex.payload
     gbs = algo / (ms * 1e-3) / 1e9
     log(f"batched: {ms*1e3:.1f} us/step, {gbs:.1f} GB/s ({100*gbs/HBM_PEAK_GBS:.1f}% of HBM peak), "
@@ -784,7 +798,7 @@ def main():
         zp.graph_replay(GRAPH_STEPS, stream)
         torch.cuda.synchronize()
         ez.verify(torch)
-        zms = timed_steps(torch, zp, args.steps, args.warmup, stream)
+        zms, _ = timed_steps(torch, zp, args.steps, args.warmup, stream)
         ez.verify(torch)
         zipf = {"ms_per_step": round(zms, 5), "GBps": round(2 * ez.payload / (zms * 1e-3) / 1e9, 2),
                 "max_recv_bytes": int(ez.rtot.max()) * 4, "payload_bytes": ez.payload,
@@ -849,7 +863,7 @@ def main():
                                "Alltoallv(keys): plan, scan and copy work, replayed from hipGraphs of "
                                "10 steps in which each step's copy kernel also plans the next step",
                    "ranks": P, "keys_per_rank": n, "payload_bytes": ex.payload,
-                   "tune": tune_used, "graph_steps": GRAPH_STEPS,
+                   "tune": tune_used, "graph_steps": GRAPH_STEPS, "warmup_steps_run": warm_run,
                    "kernel_us": {k: round(v, 2) for k, v in kern.items()},
                    "pipelined_kernel_us": {k: round(v, 2) for k, v in pipe.items()} if pipe else None},
         "roofline": {"bound": "hbm", "kernel": dom_name,

@@ -9,8 +9,8 @@
 //
 // The reference runs this as P-1 point-to-point messages into a root buffer of P*N
 // elements and a serial fold on the root (then a Bcast for Allreduce).  Here the step
-// is one grid over the element range: each lane owns a 16-byte column, folds it over
-// the P rank buffers in rank order with 16 loads in flight (fold.h), and stores the
+// is one grid over the element range: each lane owns U 16-byte columns, folds them over
+// the P rank buffers in rank order with D ranks' loads in flight (fold.h), and stores the
 // result to the root's buffer or to every rank's.  HBM traffic = the algorithmic bytes:
 // P * N * s read, N * s (Reduce) or P * N * s (Allreduce) written.
 #pragma once

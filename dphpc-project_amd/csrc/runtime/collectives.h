@@ -133,23 +133,29 @@ struct ReduceFunctor {
                 return;
             }
         }
-        // The slice in 16-byte columns, one per lane, each folded over the P inputs in rank
-        // order with kFoldDepth loads in flight (fold.h); the elements past the last whole
-        // column one per lane.  1- and 2-byte types go element by element (a 16-byte column
-        // of them unpacks into 8-16 values per lane).
+        // The slice in tiles of kFoldU 16-byte columns per lane (kFoldU KiB per rank buffer
+        // and visit), each folded over the P inputs in rank order with kFoldD ranks' loads
+        // in flight (fold.h); the elements past the last whole column one per lane.  1- and
+        // 2-byte types go element by element (a 16-byte column of them unpacks into 8-16
+        // values per lane).
         const PtrTable src{(const char*)&slots[0].sbuf, (int64_t)sizeof(CollSlot)};
         const PtrTable dst{(const char*)&slots[0].rbuf, (int64_t)sizeof(CollSlot)};
         const uint64_t one = root >= 0 ? (uint64_t)(uintptr_t)slots[root].rbuf : 0;
         const int64_t b0 = lo * (int64_t)sizeof(T);
         const int64_t nv = sizeof(T) >= 4 ? (hi - lo) * (int64_t)sizeof(T) / 16 : 0;
-        for (int64_t v0 = 0; v0 < nv; v0 += kWave) {
+        constexpr int64_t kTile = (int64_t)kWave * kFoldU;
+        for (int64_t v0 = 0; v0 < nv; v0 += kTile) {
             const int64_t v = v0 + lane();
-            const bool valid = v < nv;
+            uint32_t valid = 0;
+#pragma unroll
+            for (int u = 0; u < kFoldU; u++)
+                if (v + (int64_t)kWave * u < nv) valid |= 1u << u;
             const uint64_t off = (uint64_t)(b0 + 16 * v);
             if constexpr (sizeof(T) >= 4) {
-                const u32x4 acc = fold_column<T, OP>(src, P, off, valid);
-                if (nt) fold_store<true>(dst, P, one, off, valid, acc);
-                else fold_store<false>(dst, P, one, off, valid, acc);
+                u32x4 acc[kFoldU];
+                fold_columns<T, OP, kFoldU, kFoldD>(src, P, off, valid, acc);
+                if (nt) fold_store_cols<true, kFoldU>(dst, P, one, off, valid, acc);
+                else fold_store_cols<false, kFoldU>(dst, P, one, off, valid, acc);
             }
         }
         fold_scalar_tail<T, OP>(src, P, dst, P, one, lo + nv * 16 / (int64_t)sizeof(T), hi);

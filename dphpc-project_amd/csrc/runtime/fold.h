@@ -1,13 +1,12 @@
 // Element-wise reduction of P rank buffers in the reference's fold order
 // (acc = in_0; acc = op(in_r, acc) for r = 1..P-1, gpu_mpi/mpi.cu:209-225, operator bodies
-// operators.cu:73-151), one 16-byte column per lane.
+// operators.cu:73-151), in 16-byte columns.
 //
 // Every element keeps its own left-to-right fold, so results are bit-exact with the
 // reference's generic path for every type, FP64 included; the parallelism is over
-// elements only.  A lane's column is loaded from the P sources in rank order with
-// kFoldDepth loads in flight (a ring of 16-byte registers, refilled DEPTH ranks ahead),
-// which keeps a wave's 64 columns (1 KiB per rank) streaming from HBM instead of waiting
-// one load at a time.  Source and destination addresses come from tables of per-rank
+// elements only.  A lane's columns are loaded from the P sources in rank order with D
+// ranks' loads in flight (a ring of 16-byte registers, refilled D ranks ahead), which
+// keeps a wave streaming from HBM instead of waiting one load at a time.  Source and destination addresses come from tables of per-rank
 // pointers (a stride apart); a wave loads 64 of them at once into its lanes and reads
 // each back with readlane.
 #pragma once
@@ -16,8 +15,6 @@
 #include "copy.h"
 
 namespace gpumpi {
-
-constexpr int kFoldDepth = 16;
 
 // Per-rank pointer tables: entry r at base + r * stride (a uint64_t address).
 struct PtrTable {
@@ -93,13 +90,10 @@ __device__ __forceinline__ void fold_columns(const PtrTable& src, int P, uint64_
     }
 }
 
-// One column per lane (the rank grid's slices), kFoldDepth ranks in flight.
-template <class T, int OP>
-__device__ __forceinline__ u32x4 fold_column(const PtrTable& src, int P, uint64_t off, bool valid) {
-    u32x4 acc[1];
-    fold_columns<T, OP, 1, kFoldDepth>(src, P, off, valid ? 1u : 0u, acc);
-    return acc[0];
-}
+// The rank grid's slices: 2 columns per lane (2 KiB of each rank buffer per visit), 8
+// ranks in flight -- the shape that the batched kernel measured fastest at 1 MiB per rank.
+constexpr int kFoldU = 2;
+constexpr int kFoldD = 8;
 
 // Stores a lane's folded column to every destination of the table (ndst of them), or to
 // one destination address when `one` is non-zero.
