@@ -39,7 +39,7 @@ class Config(ctypes.Structure):
     _fields_ = [
         ("num_ranks", ctypes.c_int32), ("ranks_per_block", ctypes.c_int32),
         ("ring_slots", ctypes.c_int32), ("max_requests", ctypes.c_int32),
-        ("max_unexpected", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("max_unexpected", ctypes.c_int32), ("bcast_push_min", ctypes.c_int32),
         ("heap_bytes", ctypes.c_uint64), ("stack_bytes", ctypes.c_uint64),
         ("timeout_s", ctypes.c_double),
     ]

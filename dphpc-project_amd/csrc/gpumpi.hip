@@ -231,6 +231,7 @@ void gpumpi_config_default(gpumpi_config* c) {
     c->max_unexpected = 1024;
     c->heap_bytes = 1ull << 30;
     c->timeout_s = 20.0;
+    c->bcast_push_min = kBcastPushMinDefault;
 }
 
 int gpumpi_device_info(int32_t* num_cus, int32_t* wallclock_khz, int32_t* max_waves_per_cu) {
@@ -276,6 +277,7 @@ int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
         if (cfg.max_unexpected <= 0) cfg.max_unexpected = 1024;
         if (cfg.heap_bytes == 0) cfg.heap_bytes = 1ull << 30;
         if (cfg.timeout_s <= 0) cfg.timeout_s = 20.0;
+        if (cfg.bcast_push_min == 0) cfg.bcast_push_min = kBcastPushMinDefault;
     }
     if (cfg.num_ranks <= 0 || cfg.ranks_per_block > 4) {
         g_host_error = "invalid num_ranks or ranks_per_block";
@@ -301,6 +303,7 @@ int gpumpi_runtime_create(const gpumpi_config* cfg_in, gpumpi_runtime** out) {
     h.ring_slots = cfg.ring_slots;
     h.max_requests = cfg.max_requests;
     h.max_unexpected = cfg.max_unexpected;
+    h.bcast_push_min = cfg.bcast_push_min;
     h.timeout_ticks = (uint64_t)(cfg.timeout_s * (double)khz * 1000.0);
     h.wtick = 1.0 / ((double)khz * 1000.0);
     h.next_ctx = 2;

@@ -162,6 +162,8 @@ struct Runtime {
     int32_t abort_flag;   // atomic
     int32_t exit_code;    // atomic or
     int32_t err_lock;     // atomic
+    int32_t bcast_push_min;  // Bcast pushes slices from this many bytes per rank; < 0 never
+    int32_t pad0;
     uint64_t timeout_ticks;
     double wtick;         // seconds per wall_clock64 tick
     uint64_t next_ctx;    // atomic context-id allocator

@@ -52,7 +52,12 @@ typedef struct gpumpi_config {
     int32_t ring_slots;         /* envelope ring slots per rank; 0 = 256 */
     int32_t max_requests;       /* -e: request pool per rank; 0 = 1024 */
     int32_t max_unexpected;     /* unexpected-message list per rank; 0 = 1024 */
-    int32_t reserved0;
+    int32_t bcast_push_min;     /* MPI_Bcast of at least this many bytes per rank (buffer
+                                   bytes / ranks) runs as a push: every rank stores one
+                                   slice of the root's buffer into all receivers; smaller
+                                   ones are pulled by each receiver.  0 = 512 with at
+                                   least 512 KiB in all (the measured crossover on 64 and
+                                   1024 ranks), < 0 never */
     uint64_t heap_bytes;        /* -p: device heap arena; 0 = 1 GiB */
     uint64_t stack_bytes;       /* -s: per-lane stack limit; 0 = runtime default */
     double timeout_s;           /* bound on every device-side wait; 0 = 20 s */

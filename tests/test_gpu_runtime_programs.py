@@ -117,14 +117,17 @@ def test_illegal_operator_pair_is_an_error(dt, op, pkg, oracle, torch_gpu):
     assert "not allowed" in err, err
 
 
-@pytest.mark.parametrize("rpb", [1, 4])
-def test_lds_buffers(rpb, pkg, torch_gpu):
+@pytest.mark.parametrize("rpb,push", [(1, 0), (4, 0), (4, 1)])
+def test_lds_buffers(rpb, push, pkg, torch_gpu):
+    """Point-to-point and collectives on LDS buffers; push=1 sends the Bcast into LDS
+    through the push schedule (a staged buffer stored into by the other ranks)."""
     from dphpc_project_amd import programs as pr
     torch = torch_gpu
     P = 10
-    print(f"[gpu] lds buffers P={P} rpb={rpb}", flush=True)
+    print(f"[gpu] lds buffers P={P} rpb={rpb} push={push}", flush=True)
     ok = torch.zeros(P, dtype=torch.int32, device="cuda")
-    with pkg.Runtime(P, ranks_per_block=rpb, heap_bytes=64 << 20, timeout_s=20.0) as rt:
+    with pkg.Runtime(P, ranks_per_block=rpb, heap_bytes=64 << 20, timeout_s=20.0,
+                     bcast_push_min=1 if push else 0) as rt:
         rt.run("t_lds_buffers", pr.LdsArgs(ok.data_ptr()))
     assert ok.cpu().tolist() == [1] * P
 

@@ -97,7 +97,7 @@ def test_p2p_256_ranks(mode, nbytes, pkg, oracle, torch_gpu):
         assert np.array_equal(got, fill_bytes(oracle, seed, origin, nbytes)), r
 
 
-def run_coll(pkg, torch, P, kind, dtype, op, count, root=0, seed=0x1234):
+def run_coll(pkg, torch, P, kind, dtype, op, count, root=0, seed=0x1234, **cfg):
     from dphpc_project_amd import programs as pr
     es = 8 if dtype == MPI_DOUBLE else 4
     in_elems = count * (P if kind == "scatter" else 1)
@@ -108,7 +108,7 @@ def run_coll(pkg, torch, P, kind, dtype, op, count, root=0, seed=0x1234):
     prog_op = 0 if op == OP_SUM else 1  # the program's own encoding: 0 = MPI_SUM, 1 = MPI_MAX
     args = pr.CollArgs(seed, pr.COLL_KINDS[kind], dtype, prog_op, root, count, 1, 0,
                        inb.data_ptr(), outb.data_ptr(), in_elems * es, out_elems * es, sec.data_ptr())
-    with pkg.Runtime(P, heap_bytes=128 << 20, timeout_s=20.0) as rt:
+    with pkg.Runtime(P, heap_bytes=128 << 20, timeout_s=20.0, **cfg) as rt:
         rt.run("coll", args)
     npdt = np.float64 if dtype == MPI_DOUBLE else np.int32
     out = outb.cpu().numpy()[:P * out_elems * es].view(npdt).reshape(P, out_elems)
@@ -168,6 +168,25 @@ def test_bcast_alltoall_gather_scatter(dtype, P, count, pkg, oracle, torch_gpu):
     src = gen(oracle, dtype, seed, 0, count * P)
     for r in range(P):
         assert np.array_equal(out[r], src[r * count:(r + 1) * count])
+
+
+@pytest.mark.parametrize("dtype,P,count,root", [
+    (MPI_INT, 3, 1027, 0),          # 4108 B: 2 KiB slices, a 12-byte tail on rank 2
+    (MPI_DOUBLE, 64, 100003, 63),   # 13 KiB slices (1 KiB passes), an 8-byte tail; root pushes too
+    (MPI_DOUBLE, 5, 25001, 2),      # 40 KiB slices: two 16 KiB passes + 1 KiB passes + a tail
+    (MPI_INT, 257, (1 << 20) + 3, 128),  # full 16 KiB passes
+    (MPI_DOUBLE, 1024, 1 << 18, 5),
+])
+def test_bcast_push_and_pull(dtype, P, count, root, pkg, oracle, torch_gpu):
+    """MPI_Bcast under both schedules (bcast_push_min 1: every size pushes; -1: never) and
+    the default choice: every rank's buffer equals the root's input."""
+    print(f"[gpu] bcast push/pull dtype={dtype} P={P} count={count} root={root}", flush=True)
+    exp = gen(oracle, dtype, 0x77, root, count)
+    for pmin in (1, -1, 0):
+        out = run_coll(pkg, torch_gpu, P, "bcast", dtype, OP_SUM, count, root=root, seed=0x77,
+                       bcast_push_min=pmin)
+        bad = [r for r in range(P) if not np.array_equal(out[r], exp)]
+        assert not bad, (pmin, bad[:8])
 
 
 def test_heap_churn_on_a_full_arena(pkg, oracle, torch_gpu):
