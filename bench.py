@@ -839,8 +839,9 @@ def main():
     if os.path.exists(tf):
         try:
             tj = json.load(open(tf))
-            if tj.get("kernel_variant", "") == ("fused" if tune_used["pipeline"] in (1, 2) else "plain"):
-                traffic = tj.get("hbm_bytes_per_launch")
+            tv = tj.get("variants", {}).get("fused" if tune_used["pipeline"] in (1, 2) else "plain")
+            if tv:
+                traffic = tv.get("hbm_bytes_per_launch")
                 traffic_src = tj.get("source")
         except Exception:
             traffic = None
