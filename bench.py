@@ -291,6 +291,9 @@ def particle_exchange(pkg, torch, args):
     steps from the same initial state must end in the rank-grid program's state bit for
     bit; then 10-step hipGraphs are timed with HIP events, and the scatter kernel alone
     (events around it) gives the roofline: 2 x 64 B per particle plus two direction bytes.
+    The batched step runs twice: exchange mode 0 (the scatter writes every particle to its
+    final slot) and mode 1 (the neighbour mode's messages: migrants packed per destination,
+    then one batched point-to-point step of P x 6 pairs, gpumpi_p2p_*).
     Migrant GB/s count each migrating particle's 64 bytes read once and written once."""
     import numpy as np
     from dphpc_project_amd import programs as pr
@@ -326,48 +329,57 @@ def particle_exchange(pkg, torch, args):
     del cnt
     # both modes end in the same state; the last run (11 steps) left it in bufs[1]
     ref = (send.copy_(bufs[1]), counts[P:].clone())
+    send = None
 
-    # batched step: 11 steps from the same initial state, compared with the rank grid's
+    # batched step: 11 steps from the same initial state, compared with the rank grid's;
+    # exchange mode 0 (the scatter writes final slots) and 1 (neighbour messages through the
+    # batched point-to-point step)
     stream = torch.cuda.current_stream().cuda_stream
-    plan = pkg.ParticlePlan((gx, gy, gz), cap, seed)
     A, B = bufs
     ca = torch.zeros(P, dtype=torch.int64, device="cuda")
     cb = torch.zeros(P, dtype=torch.int64, device="cuda")
-    plan.init(A.data_ptr(), ca.data_ptr(), n0, stream)
-    plan.prepare(A.data_ptr(), ca.data_ptr(), 0, stream)
-    for i in range(11):
-        x, cx, y, cy = (A, ca, B, cb) if i % 2 == 0 else (B, cb, A, ca)
-        plan.step(x.data_ptr(), cx.data_ptr(), y.data_ptr(), cy.data_ptr(), stream)
-    torch.cuda.synchronize()
-    mask = torch.arange(cap, device="cuda")[None, :] < cb[:, None]
-    same = bool(torch.equal(cb, ref[1])) and bool(torch.equal(
-        B.view(P, cap, 8)[mask].view(torch.int64), ref[0].view(P, cap, 8)[mask].view(torch.int64)))
-    del mask, ref
-    send = None
-    assert same, "batched particle step differs from the rank-grid MPI program"
-    m0 = plan.status()[0]
-    g = 10
-    plan.graph_replay(B.data_ptr(), cb.data_ptr(), A.data_ptr(), ca.data_ptr(), g, stream)  # warm-up
-    reps = 5
-    ms = time_steps(torch, lambda: plan.graph_replay(B.data_ptr(), cb.data_ptr(), A.data_ptr(),
-                                                     ca.data_ptr(), g, stream), reps) / g
-    m1, overflow, nxt = plan.status()
-    assert overflow == 0, "a rank exceeded its particle capacity"
-    per_step = (m1 - m0) / ((reps + 1) * g)
-    # kernel durations of plain steps (events around each kernel), averaged
-    plan.phase_times(True)
-    kt = np.zeros(3)
-    nk = 6
-    for i in range(nk):
-        x, cx, y, cy = (B, cb, A, ca) if i % 2 == 0 else (A, ca, B, cb)
-        plan.step(x.data_ptr(), cx.data_ptr(), y.data_ptr(), cy.data_ptr(), stream)
-        kt += np.array(plan.phase_times())
-    plan.phase_times(False)
-    kt /= nk
     nparts = P * n0
+    rg = out["alltoallv"]["ms_per_step"]
+
+    def batched(mode):
+        plan = pkg.ParticlePlan((gx, gy, gz), cap, seed)
+        plan.exchange_mode(mode)
+        plan.init(A.data_ptr(), ca.data_ptr(), n0, stream)
+        plan.prepare(A.data_ptr(), ca.data_ptr(), 0, stream)
+        for i in range(11):
+            x, cx, y, cy = (A, ca, B, cb) if i % 2 == 0 else (B, cb, A, ca)
+            plan.step(x.data_ptr(), cx.data_ptr(), y.data_ptr(), cy.data_ptr(), stream)
+        torch.cuda.synchronize()
+        mask = torch.arange(cap, device="cuda")[None, :] < cb[:, None]
+        same = bool(torch.equal(cb, ref[1])) and bool(torch.equal(
+            B.view(P, cap, 8)[mask].view(torch.int64), ref[0].view(P, cap, 8)[mask].view(torch.int64)))
+        del mask
+        assert same, f"batched particle step (exchange mode {mode}) differs from the rank-grid MPI program"
+        m0 = plan.status()[0]
+        g = 10
+        plan.graph_replay(B.data_ptr(), cb.data_ptr(), A.data_ptr(), ca.data_ptr(), g, stream)  # warm-up
+        reps = 5
+        ms = time_steps(torch, lambda: plan.graph_replay(B.data_ptr(), cb.data_ptr(), A.data_ptr(),
+                                                         ca.data_ptr(), g, stream), reps) / g
+        m1, overflow, nxt = plan.status()
+        assert overflow == 0, "a rank exceeded its particle capacity"
+        per_step = (m1 - m0) / ((reps + 1) * g)
+        # kernel durations of plain steps (events around each kernel), averaged
+        plan.phase_times(True)
+        kt = np.zeros(4)
+        nk = 6
+        for i in range(nk):
+            x, cx, y, cy = (B, cb, A, ca) if i % 2 == 0 else (A, ca, B, cb)
+            plan.step(x.data_ptr(), cx.data_ptr(), y.data_ptr(), cy.data_ptr(), stream)
+            kt += np.array(plan.phase_times())
+        plan.phase_times(False)
+        kt /= nk
+        plan.close()
+        return same, ms, per_step, kt, reps * g
+
+    same, ms, per_step, kt, timed = batched(0)
     scatter_bytes = nparts * (2 * 64 + 2)
     scatter_gbps = scatter_bytes / (kt[2] * 1e-6) / 1e9
-    plan.close()
     scatter_traffic = None
     tf = os.path.join(ROOT, "profiles", "ps_scatter_kernel_p512_2e24_traffic.json")
     if os.path.exists(tf):
@@ -375,9 +387,8 @@ def particle_exchange(pkg, torch, args):
             scatter_traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             scatter_traffic = None
-    rg = out["alltoallv"]["ms_per_step"]
     out["batched"] = {
-        "ms_per_step": round(ms, 4), "steps_timed": reps * g, "migrants_per_step": round(per_step),
+        "ms_per_step": round(ms, 4), "steps_timed": timed, "migrants_per_step": round(per_step),
         "GBps_migrants": round(2 * 64 * per_step / (ms * 1e-3) / 1e9, 2),
         "GBps_state": round(2 * 64 * nparts / (ms * 1e-3) / 1e9, 1),
         "kernel_us": {"tally": round(kt[0], 1), "place": round(kt[1], 1), "scatter": round(kt[2], 1)},
@@ -390,6 +401,26 @@ def particle_exchange(pkg, torch, args):
     log(f"particles (batched): {ms:.3f} ms/step ({rg / ms:.1f}x the rank-grid Alltoallv step), "
         f"scatter {kt[2]:.1f} us = {scatter_gbps:.0f} GB/s ({100 * scatter_gbps / HBM_PEAK_GBS:.1f} % of peak), "
         f"tally/place {kt[0]:.1f}/{kt[1]:.1f} us; same state as the rank grid: {same}")
+
+    # neighbour messages: the pack kernel writes stayers to their slots and migrants to the
+    # send area, then one batched point-to-point step moves P x 6 groups
+    same, ms, per_step, kt, timed = batched(1)
+    p2p_bytes = 2 * 64 * per_step
+    p2p_gbps = p2p_bytes / (kt[3] * 1e-6) / 1e9
+    rn = out["neighbours"]["ms_per_step"]
+    out["batched_p2p"] = {
+        "ms_per_step": round(ms, 4), "steps_timed": timed, "migrants_per_step": round(per_step),
+        "GBps_migrants": round(p2p_bytes / (ms * 1e-3) / 1e9, 2),
+        "kernel_us": {"tally": round(kt[0], 1), "place": round(kt[1], 1), "pack": round(kt[2], 1),
+                      "p2p_step": round(kt[3], 1)},
+        "p2p_step_GBps": round(p2p_gbps, 1), "p2p_step_frac_of_hbm_peak": round(p2p_gbps / HBM_PEAK_GBS, 4),
+        "matches_rank_grid_11_steps": same,
+        "vs_rank_grid_neighbours": round(rn / ms, 2),
+    }
+    log(f"particles (batched p2p): {ms:.3f} ms/step ({rn / ms:.1f}x the rank-grid neighbour step), "
+        f"pack {kt[2]:.1f} us, p2p step {kt[3]:.1f} us = {p2p_gbps:.0f} GB/s of migrants, "
+        f"tally/place {kt[0]:.1f}/{kt[1]:.1f} us; same state as the rank grid: {same}")
+    del ref
     return out
 
 

@@ -120,6 +120,12 @@ _SIGNATURES = {
     "gpumpi_particles_status": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_I32),
                                                ctypes.POINTER(ctypes.c_uint32)]),
     "gpumpi_particles_phase_times": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_double)]),
+    "gpumpi_particles_exchange_mode": (ctypes.c_int, [_P, _I32]),
+    "gpumpi_p2p_plan_create": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_P)]),
+    "gpumpi_p2p_plan_destroy": (None, [_P]),
+    "gpumpi_p2p_execute": (ctypes.c_int, [_P, _P, ctypes.c_int64, _P]),
+    "gpumpi_p2p_graph_replay": (ctypes.c_int, [_P, _P, ctypes.c_int64, _I32, _P]),
+    "gpumpi_p2p_kernel_times": (ctypes.c_int, [_P, _P, ctypes.c_int64, _I32, _P, ctypes.POINTER(ctypes.c_double)]),
     "gpumpi_serve_start": (ctypes.c_int, [ctypes.POINTER(Config), _I32, _I32, ctypes.c_double,
                                           ctypes.POINTER(_P)]),
     "gpumpi_serve_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job)]),
@@ -534,17 +540,74 @@ class ParticlePlan:
 
     def phase_times(self, enable=None):
         """enable=True/False arms/disarms event timing of step(); with no argument returns
-        [tally_us, place_us, scatter_us] of the last timed step."""
+        [tally_us, place_us, scatter_us, p2p_us] of the last timed step (p2p_us: the batched
+        point-to-point step of exchange mode 1, 0 in mode 0)."""
         if enable is not None:
             self.lib.gpumpi_particles_phase_times(self.h, 1 if enable else 0, None)
             return None
-        out = (ctypes.c_double * 3)()
+        out = (ctypes.c_double * 4)()
         self._check(self.lib.gpumpi_particles_phase_times(self.h, -1, out), "phase_times")
         return list(out)
+
+    def exchange_mode(self, mode):
+        """0: the scatter writes final slots; 1: neighbour messages through the batched
+        point-to-point step (gpumpi_particles_exchange_mode)."""
+        self._check(self.lib.gpumpi_particles_exchange_mode(self.h, int(mode)), "exchange_mode")
 
     def close(self):
         if self.h:
             self.lib.gpumpi_particles_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class P2pPair(ctypes.Structure):
+    """gpumpi_p2p_pair: one matched send/receive of a batched point-to-point step."""
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("bytes", ctypes.c_uint64)]
+
+
+class P2pPlan:
+    """Batched point-to-point step (gpumpi_p2p_*): the pairs of a step are a device array of
+    gpumpi_p2p_pair (24 bytes each: src, dst, bytes), read when the step runs."""
+
+    def __init__(self, max_pairs):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.gpumpi_p2p_plan_create(int(max_pairs), ctypes.byref(h))
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_p2p_plan_create failed ({rc})")
+        self.h = h
+
+    @staticmethod
+    def _s(stream):
+        return ctypes.c_void_p(stream) if stream else None
+
+    def _check(self, rc, what):
+        if rc != OK:
+            raise GpumpiError(f"gpumpi_p2p_{what} failed ({rc})")
+
+    def execute(self, pairs_ptr, npairs, stream=0):
+        self._check(self.lib.gpumpi_p2p_execute(self.h, pairs_ptr, int(npairs), self._s(stream)), "execute")
+
+    def graph_replay(self, pairs_ptr, npairs, steps, stream=0):
+        self._check(self.lib.gpumpi_p2p_graph_replay(self.h, pairs_ptr, int(npairs), int(steps),
+                                                     self._s(stream)), "graph_replay")
+
+    def kernel_times(self, pairs_ptr, npairs, reps=10, stream=0):
+        """Average [scan_us, copy_us] over `reps` timed steps."""
+        out = (ctypes.c_double * 2)()
+        self._check(self.lib.gpumpi_p2p_kernel_times(self.h, pairs_ptr, int(npairs), int(reps),
+                                                     self._s(stream), out), "kernel_times")
+        return list(out)
+
+    def close(self):
+        if self.h:
+            self.lib.gpumpi_p2p_plan_destroy(self.h)
             self.h = None
 
     def __del__(self):

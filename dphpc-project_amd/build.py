@@ -70,7 +70,7 @@ def build_programs(verbose=False):
 # C MPI programs built through the converter and gpumpicc (rank arrays sized for 64 ranks):
 # (output name, sources relative to tests/programs/c, extra gpumpicc arguments)
 C_PROGRAMS = [
-    ("cpi", ["cpi.c"], []),
+    ("quad", ["quad.c"], []),
     ("globals", ["globals.c"], []),
     ("coalesce", ["coalesce.c"], ["--malloc-version", "5"]),
     ("ring", ["proj/main.c", "proj/util.c"], []),

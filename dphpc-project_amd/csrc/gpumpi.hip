@@ -1292,6 +1292,131 @@ int gpumpi_a2av_autotune(gpumpi_a2av_plan* pl, void* stream, gpumpi_a2av_tune* c
 
 }  // extern "C"
 
+// ==================================================================== batched point-to-point step
+static_assert(sizeof(gpumpi_p2p_pair) == sizeof(P2pPair) && offsetof(gpumpi_p2p_pair, bytes) == offsetof(P2pPair, bytes),
+              "gpumpi_p2p_pair and the device's P2pPair differ");
+
+struct gpumpi_p2p_plan {
+    int64_t max_pairs = 0;
+    int64_t* chunk_off = nullptr;   // [max_pairs + 1]
+    int32_t grid = 0;               // copy workgroups
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    int graph_steps = 0;
+    const void* graph_pairs = nullptr;
+    int64_t graph_n = -1;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+};
+
+// The step's two kernels on stream s (no allocation, no host synchronisation: capturable).
+static int p2p_launch(gpumpi_p2p_plan* pl, const P2pPair* pairs, int64_t n, hipStream_t s, bool timed) {
+    if (timed) hipEventRecord(pl->ev[0], s);
+    p2p_scan_kernel<<<dim3(1), dim3(kP2pScanThreads), 0, s>>>(pairs, n, pl->chunk_off);
+    if (timed) hipEventRecord(pl->ev[1], s);
+    p2p_copy_kernel<<<dim3((unsigned)pl->grid), dim3(kP2pThreads), 0, s>>>(pairs, n, pl->chunk_off);
+    if (timed) hipEventRecord(pl->ev[2], s);
+    return hipGetLastError() == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+static void p2p_drop_graph(gpumpi_p2p_plan* pl) {
+    if (pl->graph) hipGraphExecDestroy(pl->graph);
+    if (pl->graph_def) hipGraphDestroy(pl->graph_def);
+    pl->graph = nullptr;
+    pl->graph_def = nullptr;
+    pl->graph_steps = 0;
+}
+
+static int p2p_create(int64_t max_pairs, gpumpi_p2p_plan** out) {
+    if (!out || max_pairs < 1 || max_pairs > (int64_t(1) << 32)) return GPUMPI_E_ARG;
+    int32_t cus = 0;
+    if (gpumpi_device_info(&cus, nullptr, nullptr) != GPUMPI_OK || cus <= 0) return GPUMPI_E_HIP;
+    gpumpi_p2p_plan* pl = new gpumpi_p2p_plan();
+    pl->max_pairs = max_pairs;
+    pl->grid = 4 * cus;   // 16 waves per CU
+    if (hipMalloc((void**)&pl->chunk_off, sizeof(int64_t) * (size_t)(max_pairs + 1)) != hipSuccess) {
+        delete pl;
+        return GPUMPI_E_NOMEM;
+    }
+    for (hipEvent_t& e : pl->ev) hipEventCreate(&e);
+    *out = pl;
+    return GPUMPI_OK;
+}
+
+static void p2p_destroy(gpumpi_p2p_plan* pl) {
+    if (!pl) return;
+    hipDeviceSynchronize();
+    p2p_drop_graph(pl);
+    for (hipEvent_t e : pl->ev)
+        if (e) hipEventDestroy(e);
+    hipFree(pl->chunk_off);
+    delete pl;
+}
+
+extern "C" {
+
+int gpumpi_p2p_plan_create(int64_t max_pairs, gpumpi_p2p_plan** out) { return p2p_create(max_pairs, out); }
+
+void gpumpi_p2p_plan_destroy(gpumpi_p2p_plan* pl) { p2p_destroy(pl); }
+
+int gpumpi_p2p_execute(gpumpi_p2p_plan* pl, const gpumpi_p2p_pair* pairs, int64_t npairs, void* stream) {
+    if (!pl || npairs < 0 || npairs > pl->max_pairs || (npairs > 0 && !pairs)) return GPUMPI_E_ARG;
+    if (npairs == 0) return GPUMPI_OK;
+    return p2p_launch(pl, (const P2pPair*)pairs, npairs, (hipStream_t)stream, false);
+}
+
+int gpumpi_p2p_graph_replay(gpumpi_p2p_plan* pl, const gpumpi_p2p_pair* pairs, int64_t npairs,
+                            int32_t steps, void* stream) {
+    if (!pl || !pairs || npairs < 1 || npairs > pl->max_pairs || steps < 1) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (!pl->graph || pl->graph_steps != steps || pl->graph_pairs != (const void*)pairs || pl->graph_n != npairs) {
+        p2p_drop_graph(pl);
+        hipStream_t cs;
+        if (hipStreamCreate(&cs) != hipSuccess) return GPUMPI_E_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            hipStreamDestroy(cs);
+            return GPUMPI_E_HIP;
+        }
+        int rc = GPUMPI_OK;
+        for (int i = 0; i < steps && rc == GPUMPI_OK; i++) rc = p2p_launch(pl, (const P2pPair*)pairs, npairs, cs, false);
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(cs, &g);
+        hipStreamDestroy(cs);
+        if (rc != GPUMPI_OK || ee != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return GPUMPI_E_HIP;
+        }
+        pl->graph_def = g;
+        if (hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+            p2p_drop_graph(pl);
+            return GPUMPI_E_HIP;
+        }
+        pl->graph_steps = steps;
+        pl->graph_pairs = pairs;
+        pl->graph_n = npairs;
+    }
+    return hipGraphLaunch(pl->graph, s) == hipSuccess ? GPUMPI_OK : GPUMPI_E_HIP;
+}
+
+int gpumpi_p2p_kernel_times(gpumpi_p2p_plan* pl, const gpumpi_p2p_pair* pairs, int64_t npairs, int32_t reps,
+                            void* stream, double* us) {
+    if (!pl || !pairs || !us || npairs < 1 || npairs > pl->max_pairs || reps < 1) return GPUMPI_E_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    us[0] = us[1] = 0.0;
+    for (int i = 0; i < reps; i++) {
+        if (p2p_launch(pl, (const P2pPair*)pairs, npairs, s, true) != GPUMPI_OK) return GPUMPI_E_HIP;
+        if (hipEventSynchronize(pl->ev[2]) != hipSuccess) return GPUMPI_E_HIP;
+        float a = 0.f, b = 0.f;
+        hipEventElapsedTime(&a, pl->ev[0], pl->ev[1]);
+        hipEventElapsedTime(&b, pl->ev[1], pl->ev[2]);
+        us[0] += 1000.0 * a / reps;
+        us[1] += 1000.0 * b / reps;
+    }
+    return GPUMPI_OK;
+}
+
+}  // extern "C"
+
 // ==================================================================== batched particle step
 struct gpumpi_particle_plan {
     PartStepDev dev{};
@@ -1305,7 +1430,9 @@ struct gpumpi_particle_plan {
     const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
     int graph_cur = -1;       // direction buffer the captured steps start from
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int32_t exchange = 0;           // 0: the scatter writes final slots; 1: batched p2p messages
+    gpumpi_p2p_plan* p2p = nullptr; // exchange 1
 };
 
 static int ps_launch(gpumpi_particle_plan* pl, const double* in, const int64_t* cin, double* out,
@@ -1321,9 +1448,13 @@ static int ps_launch(gpumpi_particle_plan* pl, const double* in, const int64_t* 
     const dim3 chunk_grid((unsigned)((waves + 3) / 4)), rank_grid((unsigned)d.P),
         place_grid((unsigned)((d.P + kPsThreads - 1) / kPsThreads)), block(kPsThreads);
     void* kargs[] = {(void*)&d};
-    const void* scatter = pl->nontemporal ? (const void*)&ps_scatter_kernel<true>
-                                          : (const void*)&ps_scatter_kernel<false>;
-    const void* kernels[3] = {(const void*)&ps_tally_kernel, (const void*)&ps_place_kernel, scatter};
+    const bool p2p = pl->exchange == 1;
+    const void* scatter = p2p ? (pl->nontemporal ? (const void*)&ps_scatter_kernel<true, true>
+                                                 : (const void*)&ps_scatter_kernel<false, true>)
+                              : (pl->nontemporal ? (const void*)&ps_scatter_kernel<true, false>
+                                                 : (const void*)&ps_scatter_kernel<false, false>);
+    const void* place = p2p ? (const void*)&ps_place_kernel<true> : (const void*)&ps_place_kernel<false>;
+    const void* kernels[3] = {(const void*)&ps_tally_kernel, place, scatter};
     const dim3 grids[3] = {rank_grid, place_grid, chunk_grid};
     const dim3 blocks[3] = {dim3(kPsTallyThreads), block, block};
     for (int i = 0; i < 3; i++) {
@@ -1332,6 +1463,8 @@ static int ps_launch(gpumpi_particle_plan* pl, const double* in, const int64_t* 
             return GPUMPI_E_HIP;
     }
     if (timed) hipEventRecord(pl->ev[3], s);
+    if (p2p && p2p_launch(pl->p2p, d.pairs, (int64_t)d.P * 6, s, false) != GPUMPI_OK) return GPUMPI_E_HIP;
+    if (timed) hipEventRecord(pl->ev[4], s);
     pl->cur ^= 1;
     return GPUMPI_OK;
 }
@@ -1397,7 +1530,7 @@ int gpumpi_particles_plan_create(int32_t gx, int32_t gy, int32_t gz, int64_t cap
     hipMemcpy(q, cls.data(), b_cls, hipMemcpyHostToDevice); q += al(b_cls);
     d.stats = (int64_t*)q;
     d.step = (uint32_t*)(q + 4 * sizeof(int64_t));
-    for (int i = 0; i < 4; i++) hipEventCreate(&pl->ev[i]);
+    for (int i = 0; i < 5; i++) hipEventCreate(&pl->ev[i]);
     if (hipDeviceSynchronize() != hipSuccess) {
         gpumpi_particles_plan_destroy(pl);
         return GPUMPI_E_HIP;
@@ -1413,7 +1546,32 @@ void gpumpi_particles_plan_destroy(gpumpi_particle_plan* pl) {
     for (hipEvent_t e : pl->ev)
         if (e) hipEventDestroy(e);
     for (void* p : pl->allocs) hipFree(p);
+    p2p_destroy(pl->p2p);
     delete pl;
+}
+
+int gpumpi_particles_exchange_mode(gpumpi_particle_plan* pl, int32_t mode) {
+    if (!pl || mode < 0 || mode > 1) return GPUMPI_E_ARG;
+    if (mode == 1 && !pl->p2p) {
+        PartStepDev& d = pl->dev;
+        const size_t b_send = sizeof(double) * kPartFields * (size_t)d.P * (size_t)d.cap;
+        const size_t b_pairs = ((sizeof(P2pPair) * (size_t)d.P * 6) + 255) & ~(size_t)255;
+        char* w = nullptr;
+        if (hipMalloc((void**)&w, b_send + b_pairs) != hipSuccess) return GPUMPI_E_NOMEM;
+        pl->allocs.push_back(w);
+        gpumpi_p2p_plan* q = nullptr;
+        const int rc = p2p_create((int64_t)d.P * 6, &q);
+        if (rc != GPUMPI_OK) return rc;
+        pl->p2p = q;
+        d.pairs = (P2pPair*)w;
+        d.send = (double*)(w + b_pairs);
+    }
+    if (pl->exchange != mode) {
+        hipDeviceSynchronize();
+        ps_drop_graph(pl);
+        pl->exchange = mode;
+    }
+    return GPUMPI_OK;
 }
 
 int gpumpi_particles_init(gpumpi_particle_plan* pl, double* parts, int64_t* counts, int64_t n0,
@@ -1505,8 +1663,8 @@ int gpumpi_particles_phase_times(gpumpi_particle_plan* pl, int32_t enable, doubl
         return GPUMPI_OK;
     }
     if (!pl->timing || !us) return GPUMPI_E_ARG;
-    if (hipEventSynchronize(pl->ev[3]) != hipSuccess) return GPUMPI_E_HIP;
-    for (int i = 0; i < 3; i++) {
+    if (hipEventSynchronize(pl->ev[4]) != hipSuccess) return GPUMPI_E_HIP;
+    for (int i = 0; i < 4; i++) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, pl->ev[i], pl->ev[i + 1]) != hipSuccess) return GPUMPI_E_HIP;
         us[i] = 1000.0 * ms;

@@ -26,11 +26,20 @@
 //
 // HBM traffic per step: the particles read once and written once (2 x 64 B each) plus
 // 2 B of direction bytes; everything else is O(P * chunks).
+//
+// Exchange mode 1 keeps the MPI program's neighbour mode (Isend/Irecv of each rank's
+// migrants toward each neighbour) as messages: the place kernel writes one batched
+// point-to-point pair per (source, destination) group -- from the source's send area to the
+// group's slots in the destination slab -- the scatter kernel (PACK) writes stayers to their
+// slots and migrants, by class, into the sender's send area, and the batched point-to-point
+// step (batched_p2p.h) moves the groups.  The result is mode 0's bit for bit; the migrants
+// cost 2 x 64 B more each (written to the send area, then copied).
 #pragma once
 
 #include "philox.h"
 #include "common.h"
 #include "copy.h"
+#include "batched_p2p.h"
 
 namespace gpumpi {
 
@@ -101,6 +110,8 @@ struct PartStepDev {
     int64_t* base;           // [P][8]: first slot of (source, class) in its destination slab
     uint32_t* step;          // [0]: the step dir_in belongs to (the place kernel advances it)
     int64_t* stats;          // [0] migrants since prepare, [1] overflow in this step, [2] sticky
+    double* send;            // mode 1: [P][cap][8] migrants of each rank, by class
+    P2pPair* pairs;          // mode 1: [P][6] messages of the step, by destination
 };
 
 __device__ __forceinline__ int64_t ps_count_of(const PartStepDev& a, int r) {
@@ -266,6 +277,7 @@ __global__ __launch_bounds__(kPsTallyThreads) void ps_tally_kernel(PartStepDev a
 // j, in ascending s, s's migrants toward j (one class of s reaches j).  The sources of j
 // are its own neighbours: grid_neighbour(s, d) == j exactly when s is j's neighbour in the
 // opposite direction.
+template <bool P2P>
 __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
     const int j = (int)((int64_t)blockIdx.x * kPsThreads + threadIdx.x);
     if (j == 0) a.step[0] += 1u;
@@ -305,7 +317,21 @@ __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
     int64_t acc = stay;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        if (i > 0 && src[i] == src[i - 1]) continue;   // a rank reached in two directions
+        const bool dup = i > 0 && src[i] == src[i - 1];   // a rank reached in two directions
+        if constexpr (P2P) {
+            // the message of group (src[i], class cl[i]): its place in the source's send
+            // area (the classes before it come first) and in j's slab
+            P2pPair m{nullptr, nullptr, 0};
+            if (!dup && cnt[i] > 0) {
+                int64_t sb = 0;
+                for (int y = 0; y < cl[i]; y++) sb += a.tot[(int64_t)src[i] * kPsClasses + y];
+                m.src = (const char*)(a.send + ((int64_t)src[i] * a.cap + sb) * kPartFields);
+                m.dst = (char*)(a.out + ((int64_t)j * a.cap + acc) * kPartFields);
+                m.bytes = (uint64_t)cnt[i] * kPartFields * sizeof(double);
+            }
+            a.pairs[(int64_t)j * 6 + i] = m;
+        }
+        if (dup) continue;
         a.base[(int64_t)src[i] * kPsClasses + cl[i]] = acc;
         acc += cnt[i];
     }
@@ -313,19 +339,31 @@ __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
     if (acc > a.cap) {
         a.stats[1] = 1;
         a.stats[2] = 1;
+        if constexpr (P2P)
+            for (int i = 0; i < 6; i++) a.pairs[(int64_t)j * 6 + i].bytes = 0;
     }
     atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(acc - stay));
 }
 
-// Moves one chunk's particles.  The wavefront reads the chunk's 64 particles of a group as
+// Moves one chunk's particles (PACK: stayers to their slots, migrants into the rank's send
+// area by class; their direction bytes still go beside their final slots).  The wavefront
+// reads the chunk's 64 particles of a group as
 // four fully coalesced 1 KiB rows (lane l holds quarter l % 4 of particle 16 q + l / 4 in
 // row q); lane l also decides particle l of the group (its class, its rank within the class
 // by ballots, its final slot and its next direction), and the slots reach the lanes
 // holding the data by ds_bpermute.  Stores of consecutive particles to consecutive slots
 // form 1 KiB rows as well.
-template <bool NT>
+template <bool NT, bool PACK>
 __global__ __launch_bounds__(kPsThreads) void ps_scatter_kernel(PartStepDev a) {
     constexpr int G = kPsChunk / kWave;
+    if constexpr (PACK) {
+        // a step that overflows anywhere keeps only the stayers: no message moves
+        if (a.stats[1] != 0) {
+            const int64_t nt = (int64_t)gridDim.x * kPsThreads;
+            for (int64_t i = (int64_t)blockIdx.x * kPsThreads + threadIdx.x; i < (int64_t)a.P * 6; i += nt)
+                a.pairs[i].bytes = 0;
+        }
+    }
     const int64_t w = ((int64_t)blockIdx.x * kPsThreads + threadIdx.x) >> 6;
     if (w >= (int64_t)a.P * a.nchunks) return;
     const int r = (int)(w / a.nchunks);
@@ -359,13 +397,21 @@ __global__ __launch_bounds__(kPsThreads) void ps_scatter_kernel(PartStepDev a) {
     const uint32_t next_step = a.step[0];
     // slot cursors per class: destination slab start + group start + chunk offset
     int64_t cur[kPsStay + 1];
+    int64_t wcur[kPsStay];   // PACK: migrants' slots in the send area
     {
         const int x = l < kPsClasses ? l : 0;
         const int dest = x < 6 ? a.nbr_tab[(int64_t)r * kPsClasses + x] : r;
-        int64_t cv = (int64_t)dest * a.cap + a.base[(int64_t)r * kPsClasses + x] +
-                     a.chunk_off[w * kPsClasses + x];
+        const int32_t off = a.chunk_off[w * kPsClasses + x];
+        int64_t cv = (int64_t)dest * a.cap + a.base[(int64_t)r * kPsClasses + x] + off;
 #pragma unroll
         for (int c = 0; c <= kPsStay; c++) cur[c] = (int64_t)readlane64((uint64_t)cv, c);
+        if constexpr (PACK) {
+            // the send area holds the classes in order, each class in chunk order
+            const int32_t t6 = l < 6 ? (int32_t)a.tot[(int64_t)r * kPsClasses + l] : 0;
+            const int64_t sv = (int64_t)r * a.cap + (ps_wave_scan(t6) - t6) + off;
+#pragma unroll
+            for (int c = 0; c < kPsStay; c++) wcur[c] = (int64_t)readlane64((uint64_t)sv, c);
+        }
     }
 
     u32x4* dst = (u32x4*)a.out;
@@ -374,12 +420,20 @@ __global__ __launch_bounds__(kPsThreads) void ps_scatter_kernel(PartStepDev a) {
         // lane l: particle l of the group
         const int d = k0 + g * kWave + l < n ? (dirb[g] > kPsStay ? kPsStay : (int)dirb[g]) : kPsSkip;
         const int c = d == kPsSkip ? kPsSkip : ps_class_of(cls, d);
-        int64_t slot = -1;
+        int64_t slot = -1, wslot = -1;   // final slot; slot the data is written to
 #pragma unroll
         for (int x = 0; x <= kPsStay; x++) {
             const uint64_t m = ballot(c == x);
-            if (c == x) slot = cur[x] + (int64_t)ps_lanes_below(m);
+            const int64_t below = (int64_t)ps_lanes_below(m);
+            if (c == x) {
+                slot = cur[x] + below;
+                wslot = slot;
+                if constexpr (PACK)
+                    if (x < kPsStay) wslot = wcur[x] + below;
+            }
             cur[x] += __popcll(m);
+            if constexpr (PACK)
+                if (x < kPsStay) wcur[x] += __popcll(m);
         }
         int code = d;
         if (c == kPsSkip || (dropped && c != kPsStay)) code = kPsSkip;
@@ -389,8 +443,8 @@ __global__ __launch_bounds__(kPsThreads) void ps_scatter_kernel(PartStepDev a) {
             const int nd = particle_dir(a.seed, next_step, (uint64_t)idd);
             a.dir_out[slot] = (uint8_t)(nd < 0 ? kPsStay : nd);
         }
-        const uint32_t lo = (uint32_t)(uint64_t)slot;
-        const uint32_t hi = ((uint32_t)((uint64_t)slot >> 32) & 0x00FFFFFFu) | ((uint32_t)code << 24);
+        const uint32_t lo = (uint32_t)(uint64_t)wslot;
+        const uint32_t hi = ((uint32_t)((uint64_t)wslot >> 32) & 0x00FFFFFFu) | ((uint32_t)code << 24);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int from = (q * 16 + (l >> 2)) << 2;
@@ -416,7 +470,10 @@ __global__ __launch_bounds__(kPsThreads) void ps_scatter_kernel(PartStepDev a) {
                 else { val.x = (uint32_t)bits; val.y = (uint32_t)(bits >> 32); }
             }
             const uint64_t ps = ((uint64_t)(phi & 0x00FFFFFFu) << 32) | plo;
-            u32x4* p = dst + ps * 4 + part;
+            u32x4* base = dst;
+            if constexpr (PACK)
+                if (pd < kPsStay) base = (u32x4*)a.send;
+            u32x4* p = base + ps * 4 + part;
             if (NT) __builtin_nontemporal_store(val, p);
             else *p = val;
         }

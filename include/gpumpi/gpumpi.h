@@ -18,6 +18,9 @@
  *   gpumpi_last_error ...... CudaMPI::recordError / printLastError (gpu_libc/assert.cu:24-71)
  *   gpumpi_a2av_*  ......... no reference counterpart: the batched persistent-kernel
  *                            MPI_Alltoallv step (gpu_mpi/mpi.cu:363-408 semantics)
+ *   gpumpi_p2p_* ........... the matched MPI_Isend/MPI_Irecv pairs of one step
+ *                            (gpu_libc/cuda_mpi.cu:65-118, gpu_mpi/mpi.cu:617-645) as one
+ *                            batched, graph-replayable step
  *   gpumpi_particles_* ..... no reference counterpart: the batched particle step (the
  *                            particle variant of the uneven exchange)
  *   gpumpi_serve_*, gpumpi_run_job ... no reference counterpart: the device-resident job
@@ -259,8 +262,45 @@ int gpumpi_particles_graph_replay(gpumpi_particle_plan* plan, double* a, int64_t
 int gpumpi_particles_status(gpumpi_particle_plan* plan, int64_t* moved, int32_t* overflow,
                             uint32_t* next_step);
 /* enable = 1/0 arms/disarms HIP events around the kernels of gpumpi_particles_step;
- * -1 reads us[3] = {tally, place, scatter} of the last timed step. */
+ * -1 reads us[4] = {tally, place, scatter, batched p2p step (0 in exchange mode 0)} of the
+ * last timed step. */
 int gpumpi_particles_phase_times(gpumpi_particle_plan* plan, int32_t enable, double* us);
+/* How a step moves its migrants.  0 (default): the scatter kernel writes every particle to
+ * its final slot.  1: the MPI program's neighbour mode (Isend/Irecv of each rank's migrants
+ * toward each neighbour) -- the scatter packs each rank's migrants by destination into a
+ * send area, and one batched point-to-point step (gpumpi_p2p_*, one pair per source and
+ * destination) moves them into the destination slabs.  Both give the same state bit for
+ * bit.  The first switch to 1 allocates the send area (P * cap * 64 bytes). */
+int gpumpi_particles_exchange_mode(gpumpi_particle_plan* plan, int32_t mode);
+
+/* ------------------------------------------------------------------------------------
+ * Batched point-to-point step: the matched MPI_Isend/MPI_Irecv pairs of one step
+ * (gpu_libc/cuda_mpi.cu:65-118, gpu_mpi/mpi.cu:617-645) in two kernels, without envelopes
+ * -- the pair list is the match.  Each pair copies `bytes` bytes from src to dst, as the
+ * matched receive would deliver them.  The list is device memory, read when the step runs,
+ * so device code earlier on the stream (or in the same graph) may write this step's pairs.
+ * Destinations must not overlap each other or a source; zero-byte pairs are allowed.
+ * ------------------------------------------------------------------------------------ */
+typedef struct gpumpi_p2p_pair {
+    const void* src;
+    void* dst;
+    uint64_t bytes;
+} gpumpi_p2p_pair;
+typedef struct gpumpi_p2p_plan gpumpi_p2p_plan;
+
+/* A plan for lists of up to max_pairs pairs (allocates 8 * (max_pairs + 1) bytes). */
+int gpumpi_p2p_plan_create(int64_t max_pairs, gpumpi_p2p_plan** out);
+void gpumpi_p2p_plan_destroy(gpumpi_p2p_plan* plan);
+/* One step over pairs[0 .. npairs) on `stream`: asynchronous, allocates nothing, may be
+ * captured into a caller's graph. */
+int gpumpi_p2p_execute(gpumpi_p2p_plan* plan, const gpumpi_p2p_pair* pairs, int64_t npairs, void* stream);
+/* `steps` steps over the same list, captured into a hipGraph once (per list, count and
+ * steps) and replayed. */
+int gpumpi_p2p_graph_replay(gpumpi_p2p_plan* plan, const gpumpi_p2p_pair* pairs, int64_t npairs,
+                            int32_t steps, void* stream);
+/* reps timed steps (synchronous); us[2] = average {scan, copy} kernel times. */
+int gpumpi_p2p_kernel_times(gpumpi_p2p_plan* plan, const gpumpi_p2p_pair* pairs, int64_t npairs,
+                            int32_t reps, void* stream, double* us);
 
 /* ------------------------------------------------------------------------------------
  * Bucket-sort caller of the exchange (the NPB IS rank() shape,

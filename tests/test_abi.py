@@ -62,7 +62,7 @@ def test_built_programs_match_the_library_layout(pkg):
     import subprocess
     import pytest
     assert pkg.load_library().gpumpi_layout_stamp() != 0
-    exe = os.path.join(ROOT, "tests", "programs", "_build", "cpi")
+    exe = os.path.join(ROOT, "tests", "programs", "_build", "quad")
     if not os.path.exists(exe):
         pytest.skip("converted programs not built")
     r = subprocess.run([exe, "---gpumpi", "-h"], capture_output=True, text=True, timeout=120)

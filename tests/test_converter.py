@@ -128,7 +128,7 @@ def test_device_libc_formatter_and_parsers_match_the_c_library(tmp_path):
     assert "fails=0" in r.stdout
 
 
-@pytest.mark.parametrize("name", ["cpi", "globals", "coalesce", "ring", "exits"])
+@pytest.mark.parametrize("name", ["quad", "globals", "coalesce", "ring", "exits"])
 def test_converted_programs_built(name):
     """build() converts and compiles every C test program for gfx950."""
     exe = os.path.join(ROOT, "tests", "programs", "_build", name)

@@ -22,11 +22,17 @@ def run(name, args, timeout=120, stdin_text=None):
 
 
 @pytest.mark.parametrize("ranks", [1, 7, 64])
-def test_cpi(ranks):
-    print(f"[gpu] converted cpi on {ranks} ranks", flush=True)
-    r = run("cpi", ["20000", "---gpumpi", "-n", str(ranks), "-p", str(64 << 20), "-t", "60"])
+def test_quadrature(ranks):
+    """Bcast of a double array, Allreduce and Gather in a converted C program: the total is
+    pi within 1e-10, and the gathered blocks added in rank order give the reduced total's
+    exact bits (MPI_Allreduce folds in rank order here)."""
+    print(f"[gpu] converted quad on {ranks} ranks", flush=True)
+    r = run("quad", ["2000", "---gpumpi", "-n", str(ranks), "-p", str(64 << 20), "-t", "60"])
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "pi is approximately 3.14159265" in r.stdout
+    assert f"quad: {ranks} blocks x 2000 panels, integral 3.14159265358" in r.stdout
+    assert ", ok" in r.stdout
+    line = [l for l in r.stdout.splitlines() if l.startswith("quad: reduced")][0].split()
+    assert line[2] == line[4], line
 
 
 @pytest.mark.parametrize("ranks", [1, 16, 64])

@@ -1,7 +1,9 @@
 """Batched particle step (gpumpi_particles_*: tally, place and scatter kernels; hipGraph
 replay) against the oracle's orc_particles_step, bit for bit: every FP64 field of every
 particle, the counts, the migrant totals and the overflow path.  At the workload's size
-(2^24 particles, 8x8x8 grid) against the rank-grid MPI program and by conservation."""
+(2^24 particles, 8x8x8 grid) against the rank-grid MPI program and by conservation.
+Exchange mode 1 (the neighbour messages through the batched point-to-point step) must
+give the same states as mode 0 in every test it runs in."""
 import numpy as np
 import pytest
 
@@ -31,11 +33,12 @@ def assert_same_state(P, got, gc, exp, ec, whole=False):
 class Chain:
     """Two device state buffers stepped by one plan."""
 
-    def __init__(self, pkg, torch, dims, cap, nontemporal=True):
+    def __init__(self, pkg, torch, dims, cap, nontemporal=True, exchange=0):
         self.torch = torch
         self.dims, self.cap = dims, cap
         self.P = dims[0] * dims[1] * dims[2]
         self.plan = pkg.ParticlePlan(dims, cap, SEED, nontemporal)
+        self.plan.exchange_mode(exchange)
         self.bufs = [torch.zeros(self.P * cap * 8, dtype=torch.float64, device="cuda") for _ in range(2)]
         self.cnts = [torch.zeros(self.P, dtype=torch.int64, device="cuda") for _ in range(2)]
         self.cur = 0
@@ -56,16 +59,17 @@ class Chain:
         return host_state(self.torch, self.bufs[self.cur], self.cnts[self.cur], self.P, self.cap)
 
 
+@pytest.mark.parametrize("exchange", [0, 1])
 @pytest.mark.parametrize("dims,cap", [((2, 2, 2), 2000), ((4, 4, 4), 2000), ((1, 2, 3), 2000),
                                       ((1, 1, 1), 2000), ((3, 1, 5), 1001), ((2, 3, 1), 1203)])
-def test_batched_steps_match_oracle(pkg, oracle, torch_gpu, dims, cap):
+def test_batched_steps_match_oracle(pkg, oracle, torch_gpu, dims, cap, exchange):
     """Every step equals the oracle's, including grids whose axes of length 1 or 2 send
     several directions to one rank or back to the sender, and slabs whose size is not a
     multiple of the 256-slot chunk or of 4 (direction bytes read at any alignment)."""
     n0, steps = min(1000, cap * 4 // 5), 4
-    print(f"[gpu] batched particles dims={dims} cap={cap}", flush=True)
+    print(f"[gpu] batched particles dims={dims} cap={cap} exchange={exchange}", flush=True)
     parts, counts = particles_init(oracle, dims, n0, SEED, cap)
-    ch = Chain(pkg, torch_gpu, dims, cap)
+    ch = Chain(pkg, torch_gpu, dims, cap, exchange=exchange)
     ch.load(parts, counts, 0)
     total = 0
     for s in range(steps):
@@ -87,12 +91,14 @@ def test_batched_init_matches_oracle(pkg, oracle, torch_gpu):
     assert_same_state(ch.P, got, gc, exp, ec, whole=True)
 
 
-def test_batched_graph_replay_continues_steps(pkg, oracle, torch_gpu):
+@pytest.mark.parametrize("exchange", [0, 1])
+def test_batched_graph_replay_continues_steps(pkg, oracle, torch_gpu, exchange):
     """Graph replays carry the step index on: 3 replays of a 2-step graph, then a plain
     step, equal 7 oracle steps."""
+    print(f"[gpu] batched particles graph replay exchange={exchange}", flush=True)
     dims, n0, cap = (4, 4, 4), 1500, 3000
     parts, counts = particles_init(oracle, dims, n0, SEED, cap)
-    ch = Chain(pkg, torch_gpu, dims, cap)
+    ch = Chain(pkg, torch_gpu, dims, cap, exchange=exchange)
     ch.load(parts, counts, 0)
     for rep in range(3):
         ch.plan.graph_replay(*ch.ptrs(0), *ch.ptrs(1), 2)
@@ -121,14 +127,17 @@ def test_batched_resume_at_later_step(pkg, oracle, torch_gpu):
     assert_same_state(ch.P, got, gc, parts, counts)
 
 
-def test_batched_overflow_keeps_stayers(pkg, oracle, torch_gpu):
+@pytest.mark.parametrize("exchange", [0, 1])
+def test_batched_overflow_keeps_stayers(pkg, oracle, torch_gpu, exchange):
     """A rank that would receive more than cap particles: like the MPI program, the step
-    writes the stayers and the counts only, and the overflow flag is raised."""
+    writes the stayers and the counts only, and the overflow flag is raised (exchange 1:
+    no message moves, so nothing lands past a slab's capacity)."""
+    print(f"[gpu] batched particles overflow exchange={exchange}", flush=True)
     dims, cap = (2, 2, 2), 512
     parts, counts = particles_init(oracle, dims, cap, SEED, cap)   # every slab full
     exp, ec, m = particles_step(oracle, dims, SEED, 0, parts, counts, allow_overflow=True)
     assert m == -1
-    ch = Chain(pkg, torch_gpu, dims, cap)
+    ch = Chain(pkg, torch_gpu, dims, cap, exchange=exchange)
     ch.load(parts, counts, 0)
     ch.step()
     got, gc = ch.state()
@@ -136,11 +145,12 @@ def test_batched_overflow_keeps_stayers(pkg, oracle, torch_gpu):
     assert ch.plan.status()[1] == 1
 
 
-@pytest.mark.parametrize("nontemporal", [True, False])
-def test_batched_full_grid_against_oracle(pkg, oracle, torch_gpu, nontemporal):
+@pytest.mark.parametrize("nontemporal,exchange", [(True, 0), (False, 0), (True, 1)])
+def test_batched_full_grid_against_oracle(pkg, oracle, torch_gpu, nontemporal, exchange):
+    print(f"[gpu] batched particles 8x8x8 nontemporal={nontemporal} exchange={exchange}", flush=True)
     dims, n0, cap = (8, 8, 8), 4096, 6144
     parts, counts = particles_init(oracle, dims, n0, SEED, cap)
-    ch = Chain(pkg, torch_gpu, dims, cap, nontemporal)
+    ch = Chain(pkg, torch_gpu, dims, cap, nontemporal, exchange)
     ch.load(parts, counts, 0)
     for s in range(3):
         parts, counts, _ = particles_step(oracle, dims, SEED, s, parts, counts)
@@ -149,15 +159,16 @@ def test_batched_full_grid_against_oracle(pkg, oracle, torch_gpu, nontemporal):
     assert_same_state(512, got, gc, parts, counts)
 
 
-def test_batched_workload_size_matches_rank_grid_program(pkg, torch_gpu):
+@pytest.mark.parametrize("exchange", [0, 1])
+def test_batched_workload_size_matches_rank_grid_program(pkg, torch_gpu, exchange):
     """2^24 particles on the 8x8x8 grid, 3 steps: the batched step and the rank-grid MPI
-    program (itself checked against the oracle) end in the same state; ids stay a
-    permutation and every particle lies in its owner's cell."""
+    program (itself checked against the oracle; neighbour mode for exchange 1) end in the
+    same state; ids stay a permutation and every particle lies in its owner's cell."""
     torch = torch_gpu
     from dphpc_project_amd import programs as pr
     dims, n0, cap, steps = (8, 8, 8), 32768, 40960, 3
     P = 512
-    ch = Chain(pkg, torch, dims, cap)
+    ch = Chain(pkg, torch, dims, cap, exchange=exchange)
     ch.plan.init(*ch.ptrs(0), n0)
     ch.plan.prepare(*ch.ptrs(0), 0)
     for _ in range(steps):
@@ -169,10 +180,10 @@ def test_batched_workload_size_matches_rank_grid_program(pkg, torch_gpu):
     cnt = torch.zeros(4 * P * P, dtype=torch.int32, device="cuda")
     moved = torch.zeros(P, dtype=torch.int64, device="cuda")
     sec = torch.zeros(P, dtype=torch.float64, device="cuda")
-    args = pr.ParticleArgs(SEED, 8, 8, 8, 0, steps, 0, n0, cap,
+    args = pr.ParticleArgs(SEED, 8, 8, 8, exchange, steps, 0, n0, cap,
                            (pr._P * 2)(bufs[0].data_ptr(), bufs[1].data_ptr()), send.data_ptr(),
                            counts.data_ptr(), cnt.data_ptr(), moved.data_ptr(), sec.data_ptr())
-    print("[gpu] rank-grid particles 8x8x8 n0=32768 steps=3", flush=True)
+    print(f"[gpu] rank-grid particles 8x8x8 n0=32768 steps=3 mode={exchange}", flush=True)
     with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=60.0) as rt:
         rt.run("particles", args)
     torch.cuda.synchronize()
