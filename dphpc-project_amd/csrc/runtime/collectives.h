@@ -343,6 +343,8 @@ __device__ int coll_reduce(const void* sendbuf, void* recvbuf, int64_t count, in
 }
 
 // ------------------------------------------------------------------ Alltoall(v)
+constexpr int64_t kAlltoallLaneBytes = 16;   // blocks up to this size: pulled lane by lane
+
 // counts/displs are in elements of the respective datatype (gpu_mpi/mpi.cu:363-408).
 __device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int* sdispls,
                               int sdt, void* recvbuf, const int* rcounts, const int* rdispls,
@@ -379,7 +381,47 @@ __device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int
     uint64_t seq = coll_begin(c, me);
     coll_publish(c, me, seq, sb, rb, sc, sd, se, uniform_scount, 0, 0);
     if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
-    for (int k = 1; k <= P; k++) {
+    // Blocks of at most kAlltoallLaneBytes into a global receive buffer: lane l pulls the
+    // blocks of sources l, l + 64, ... itself (the slot fields are read per lane, after the
+    // acquire of coll_wait_all), instead of one wave-wide copy per source -- 16 sources per
+    // lane on 1024 ranks instead of 1024 copies one after the other.  The closing barrier's
+    // release waits for these stores and its acquire drops this CU's L1 lines, so every
+    // lane of the rank then reads every block.  A private receive buffer (staging failed)
+    // keeps the wave path: a lane's store would reach only its own copy.
+    bool small = !needs_staging(rb);
+    if (small) {
+        int64_t mx = uniform ? (int64_t)uniform_rcount * re : 0;
+        if (!uniform)
+            for (int i = lane(); i < P; i += kWave) mx = rcounts[i] * (int64_t)re > mx ? rcounts[i] * (int64_t)re : mx;
+        small = wave_max(mx) <= kAlltoallLaneBytes;
+    }
+    if (small) {
+        for (int s = lane(); s < P; s += kWave) {
+            const CollSlot* src = &c->slots[s];
+            uint64_t n, soff, doff;
+            if (uniform) {
+                n = (uint64_t)uniform_rcount * re;
+                soff = (uint64_t)me * (uint64_t)src->a1 * (uint64_t)src->a0;
+                doff = (uint64_t)s * n;
+            } else {
+                n = (uint64_t)rcounts[s] * re;
+                soff = (uint64_t)src->displs[me] * (uint64_t)src->a0;
+                doff = (uint64_t)rdispls[s] * re;
+            }
+            const char* sp = (const char*)src->sbuf + soff;
+            char* dp = (char*)rb + doff;
+            if (n == 4 && (((uintptr_t)sp | (uintptr_t)dp) & 3) == 0) {
+                *(uint32_t*)dp = *(const uint32_t*)sp;
+            } else if (n == 16 && (((uintptr_t)sp | (uintptr_t)dp) & 15) == 0) {
+                *(u32x4*)dp = *(const u32x4*)sp;
+            } else if (n == 8 && (((uintptr_t)sp | (uintptr_t)dp) & 7) == 0) {
+                *(uint64_t*)dp = *(const uint64_t*)sp;
+            } else {
+                for (uint64_t b = 0; b < n; b++) dp[b] = sp[b];
+            }
+        }
+    }
+    for (int k = 1; k <= P && !small; k++) {
         const int s = (me + k) % P;
         const CollSlot* src = &c->slots[s];
         uint64_t n, soff, doff;
