@@ -601,7 +601,13 @@ def serve_latency(pkg, torch, args):
     """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
     the host, for one plain rank-grid launch per job and for the device-resident server.
     The plain phases run first (no server grid resident), then one server serves every
-    arrival rate; each served result is compared with the same job run alone."""
+    arrival rate; each served result is compared with the same job run alone.
+
+    Where the tail comes from: a served job's execution is its device admission ->
+    completion time (the server's 100 MHz wall-clock stamps), a plain job's is its launch
+    timed alone with HIP events (the capacity run); queueing = latency - execution.  Each
+    rate reports both percentiles, and the slowest 1 % of served jobs at 1x load are listed
+    with their type and size."""
     import numpy as np
     heap = 16 << 30
     budget = int(0.9 * heap)
@@ -612,11 +618,17 @@ def serve_latency(pkg, torch, args):
     for j in jobs[:8]:
         rt.run_job(j, res.data_ptr() + 64 * j.id, stream.cuda_stream)
     torch.cuda.synchronize()
+    ev0 = [torch.cuda.Event(enable_timing=True) for _ in jobs]
+    ev1 = [torch.cuda.Event(enable_timing=True) for _ in jobs]
     t0 = time.perf_counter()
     for j in jobs:
+        ev0[j.id].record(stream)
         rt.run_job(j, res.data_ptr() + 64 * j.id, stream.cuda_stream)
+        ev1[j.id].record(stream)
     torch.cuda.synchronize()
     plain_capacity = len(jobs) / (time.perf_counter() - t0)
+    plain_exec = np.array([ev0[k].elapsed_time(ev1[k]) for k in range(len(jobs))])   # ms
+    del ev0, ev1
     plain_raw = res.cpu().numpy().reshape(len(jobs), 64)
     plain_res = [pkg.JobResult.from_buffer_copy(plain_raw[k].tobytes()) for k in range(len(jobs))]
     plain_failed = sum(1 for p in plain_res if p.status != 1)
@@ -660,9 +672,13 @@ def serve_latency(pkg, torch, args):
             if now - arrivals[-1] > 120:
                 break
         span = time.perf_counter() - start
+        ids = np.array(list(lat_p.keys()), dtype=np.int64)
         lp = np.array(list(lat_p.values())) * 1e3
+        qp = lp - plain_exec[ids]
         plain_rows[frac] = {"p50_ms": pct(lp, 50), "p99_ms": pct(lp, 99),
-                            "jobs_per_s": round(len(lp) / span, 1), "completed": len(lp)}
+                            "jobs_per_s": round(len(lp) / span, 1), "completed": len(lp),
+                            "exec_p50_ms": pct(plain_exec[ids], 50), "exec_p99_ms": pct(plain_exec[ids], 99),
+                            "queue_p50_ms": pct(qp, 50), "queue_p99_ms": pct(qp, 99)}
     torch.cuda.synchronize()
     rt.close()
 
@@ -672,7 +688,7 @@ def serve_latency(pkg, torch, args):
     mismatches, served_failed = 0, 0
     for frac in args.serve_rates:
         arrivals = arrivals_for(frac)
-        lat_s = {}
+        lat_s, exec_s = {}, {}
         start = time.perf_counter()
         last_log = start
         i = 0
@@ -684,6 +700,7 @@ def serve_latency(pkg, torch, args):
                 i += 1
             for r in srv.poll():
                 lat_s[r.id] = (time.perf_counter() - start) - arrivals[r.id]
+                exec_s[r.id] = (r.t_done_ticks - r.t_admit_ticks) * 1e-5   # 100 MHz ticks -> ms
                 p = plain_res[r.id]
                 if r.status != 1:
                     served_failed += 1
@@ -695,14 +712,43 @@ def serve_latency(pkg, torch, args):
             if now - arrivals[-1] > 120:
                 break
         span = time.perf_counter() - start
+        ids = np.array(list(lat_s.keys()), dtype=np.int64)
         ls = np.array(list(lat_s.values())) * 1e3
+        es = np.array([exec_s[k] for k in ids])
+        qs = ls - es
         row = {"rate_fraction": frac, "rate_jobs_per_s": round(frac * plain_capacity, 1),
                "served": {"p50_ms": pct(ls, 50), "p99_ms": pct(ls, 99),
-                          "jobs_per_s": round(len(ls) / span, 1), "completed": len(ls)},
+                          "jobs_per_s": round(len(ls) / span, 1), "completed": len(ls),
+                          "exec_p50_ms": pct(es, 50), "exec_p99_ms": pct(es, 99),
+                          "queue_p50_ms": pct(qs, 50), "queue_p99_ms": pct(qs, 99)},
                "plain": plain_rows[frac]}
+        if frac == 1.0 and len(ls):
+            # the slowest 1 % of served jobs: what they are and where their time went
+            cut = np.percentile(ls, 99)
+            slow = [int(k) for k, v in zip(ids, ls) if v >= cut]
+            slow.sort(key=lambda k: -lat_s[k])
+            tail = []
+            for k in slow:
+                j = jobs[k]
+                tail.append({"type": "dot" if j.type == 0 else "alltoall", "ranks": j.ranks,
+                             "size": int(j.size), "latency_ms": round(lat_s[k] * 1e3, 2),
+                             "exec_ms": round(exec_s[k], 2), "plain_exec_ms": round(float(plain_exec[k]), 2)})
+            big = sorted(range(len(jobs)), key=lambda k: -plain_exec[k])[:len(slow)]
+            out["served_tail_at_1x"] = {
+                "size_unit": "dot: FP64 vector length; alltoall: bytes per pair",
+                "jobs": tail,
+                "exec_share_of_latency": round(float(sum(t["exec_ms"] for t in tail) /
+                                                     max(1e-9, sum(t["latency_ms"] for t in tail))), 3),
+                "also_longest_alone": len(set(slow) & set(big)),
+            }
+            log(f"serve tail at 1x: {len(tail)} jobs >= p99, execution {100 * out['served_tail_at_1x']['exec_share_of_latency']:.0f} % "
+                f"of their latency; {out['served_tail_at_1x']['also_longest_alone']} of them are among the "
+                f"{len(slow)} longest jobs run alone")
         log(f"serve rate {frac}x ({frac * plain_capacity:.0f}/s): served p50 "
-            f"{row['served']['p50_ms']} ms p99 {row['served']['p99_ms']} ms; plain p50 "
-            f"{row['plain']['p50_ms']} ms p99 {row['plain']['p99_ms']} ms")
+            f"{row['served']['p50_ms']} ms p99 {row['served']['p99_ms']} ms (exec p99 "
+            f"{row['served']['exec_p99_ms']}, queue p99 {row['served']['queue_p99_ms']}); plain p50 "
+            f"{row['plain']['p50_ms']} ms p99 {row['plain']['p99_ms']} ms (exec p99 "
+            f"{row['plain']['exec_p99_ms']}, queue p99 {row['plain']['queue_p99_ms']})")
         out["rates"].append(row)
     out["server_stats"] = srv.stop()
     out["result_mismatches_vs_plain"] = mismatches
@@ -730,7 +776,7 @@ def main():
     ap.add_argument("--no-reduce", action="store_true")
     ap.add_argument("--no-sweeps", action="store_true")
     ap.add_argument("--churn-ops", type=int, default=1000000)
-    ap.add_argument("--serve-jobs", type=int, default=400)
+    ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[0.25, 0.5, 1.0, 2.0, 4.0])
     ap.add_argument("--tune", type=str, default="",

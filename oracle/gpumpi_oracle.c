@@ -98,18 +98,75 @@ void orc_fill_int(uint64_t seed, int rank, uint32_t iter, int32_t *out, int64_t 
     }
 }
 
-/* Approximate N(0,1) as Irwin-Hall(12) - 6 evaluated in exact integer
- * arithmetic, so that every platform draws bit-identical doubles. */
+/* ln, sqrt and cos(2 pi u) from +, -, *, / and exact bit manipulation only, in the
+ * device generator's order (dphpc-project_amd/csrc/runtime/philox.h), so the oracle draws
+ * the device's bits: this file is built with -ffp-contract=off like the device code, and
+ * IEEE double +, -, *, / round the same way on both.  Pinned against libm in
+ * tests/test_oracle.py (orc_det_log / orc_det_sqrt / orc_det_cos_2pi). */
+static const double DET_LN2_HI = 0.6931471805599453;
+static const double DET_LN2_LO = 2.3190468138462996e-17;
+static const double DET_HALF_PI = 1.5707963267948966;
+static const double DET_SQRT2 = 1.4142135623730951;
+
+static double det_from_bits(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+static uint64_t det_to_bits(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
+
+/* x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1), to s^23. */
+double orc_det_log(double x)
+{
+    const uint64_t b = det_to_bits(x);
+    int e = (int)((b >> 52) & 0x7FF) - 1023;
+    double m = det_from_bits((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > DET_SQRT2) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    static const int odd[11] = {21, 19, 17, 15, 13, 11, 9, 7, 5, 3, 1};
+    double p = 1.0 / 23.0;
+    for (int k = 0; k < 11; k++) p = p * s2 + 1.0 / (double)odd[k];
+    const double lnm = (2.0 * s) * p;
+    const double de = (double)e;
+    return de * DET_LN2_HI + (de * DET_LN2_LO + lnm);
+}
+
+/* Newton from 2^floor(e/2), seven steps. */
+double orc_det_sqrt(double x)
+{
+    if (x == 0.0) return 0.0;
+    const int e = (int)((det_to_bits(x) >> 52) & 0x7FF) - 1023;
+    const int h = e >= 0 ? e / 2 : -((1 - e) / 2);
+    double y = det_from_bits((uint64_t)(h + 1023) << 52);
+    for (int k = 0; k < 7; k++) y = 0.5 * (y + x / y);
+    return y;
+}
+
+/* quadrant of 4u (exact), then the Taylor series of cos and sin of (4u - q) pi/2 in
+ * Horner form: cos t = 1 - t^2/(1*2) (1 - t^2/(3*4) (... (1 - t^2/(21*22)))),
+ * sin t = t (1 - t^2/(2*3) (... (1 - t^2/(22*23)))). */
+double orc_det_cos_2pi(double u)
+{
+    const double f = 4.0 * u;
+    const int q = (int)f;
+    const double t = (f - (double)q) * DET_HALF_PI;
+    const double t2 = t * t;
+    double c = 1.0, sn = 1.0;
+    for (int k = 21; k >= 1; k -= 2) c = 1.0 - t2 * (1.0 / (double)(k * (k + 1))) * c;
+    for (int k = 22; k >= 2; k -= 2) sn = 1.0 - t2 * (1.0 / (double)(k * (k + 1))) * sn;
+    sn = t * sn;
+    return q == 0 ? c : q == 1 ? -sn : q == 2 ? -c : sn;
+}
+
+/* N(0,1) by Box-Muller on Philox block i: sqrt(-2 ln u1) cos(2 pi u2), u1 in (0, 1] from
+ * words 0-1, u2 in [0, 1) from words 2-3, 53 bits each. */
 static void orc_fill_normal_one(uint64_t seed, int rank, uint32_t iter, int64_t i, double *out)
 {
     uint32_t o[4];
-    uint64_t s = 0;
-    for (int j = 0; j < 3; j++) {
-        philox_at(seed, rank, iter, (uint64_t)(3 * i + j), ORC_STREAM_NORMAL, o);
-        s += (uint64_t)o[0] + o[1] + o[2] + o[3];
-    }
-    int64_t c = (int64_t)s - (int64_t)6 * 4294967296LL;
-    *out = (double)c * (1.0 / 4294967296.0);
+    philox_at(seed, rank, iter, (uint64_t)i, ORC_STREAM_NORMAL, o);
+    const double u1 = (double)(((((uint64_t)o[0] << 32) | o[1]) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)((((uint64_t)o[2] << 32) | o[3]) >> 11) * (1.0 / 9007199254740992.0);
+    *out = orc_det_sqrt(-2.0 * orc_det_log(u1)) * orc_det_cos_2pi(u2);
 }
 
 void orc_fill_normal(uint64_t seed, int rank, uint32_t iter, double *out, int64_t n)

@@ -94,6 +94,10 @@ int64_t orc_mcpi_hits(uint64_t seed, int rank, int64_t n);
 void orc_fill_bytes(uint64_t seed, int rank, uint32_t iter, uint8_t *buf, int64_t nbytes);
 void orc_fill_int(uint64_t seed, int rank, uint32_t iter, int32_t *out, int64_t n);
 void orc_fill_normal(uint64_t seed, int rank, uint32_t iter, double *out, int64_t n);
+/* The generator's ln, sqrt and cos(2 pi u), exported for their pins. */
+double orc_det_log(double x);
+double orc_det_sqrt(double x);
+double orc_det_cos_2pi(double u);
 /* The same streams at the element indices idx[0..n). */
 void orc_fill_int_at(uint64_t seed, int rank, uint32_t iter, const int64_t *idx, int64_t n,
                      int32_t *out);
