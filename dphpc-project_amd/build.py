@@ -79,30 +79,35 @@ C_PROGRAMS = [
 ]
 
 
+def _build_one_c_program(item, src_dir, out_dir, env, verbose):
+    name, srcs, extra = item
+    out = os.path.join(out_dir, name)
+    cmd = [sys.executable, os.path.join(HERE, "gpumpicc.py"), *[os.path.join(src_dir, s) for s in srcs],
+           "-o", out, "--build-dir", os.path.join(out_dir, name + ".src"), *extra]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError(f"gpumpicc failed to build {name}")
+    return out
+
+
 def build_c_programs(verbose=False):
+    """The converted C test programs, compiled four at a time."""
     src_dir = os.path.join(ROOT, "tests", "programs", "c")
     out_dir = os.path.join(ROOT, "tests", "programs", "_build")
     os.makedirs(out_dir, exist_ok=True)
     env = dict(os.environ, GPU_MPI_MAX_RANKS="64")
-    outs = []
-    for name, srcs, extra in C_PROGRAMS:
-        out = os.path.join(out_dir, name)
-        cmd = [sys.executable, os.path.join(HERE, "gpumpicc.py"), *[os.path.join(src_dir, s) for s in srcs],
-               "-o", out, "--build-dir", os.path.join(out_dir, name + ".src"), *extra]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
-        if res.returncode != 0:
-            sys.stderr.write(res.stdout)
-            raise RuntimeError(f"gpumpicc failed to build {name}")
-        outs.append(out)
-    return outs
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        return list(pool.map(lambda it: _build_one_c_program(it, src_dir, out_dir, env, verbose), C_PROGRAMS))
 
 
 # NPB IS and DT, the reference's toolchain tests (toolchain_tests/npb/is_test.sh,
 # dt_test.sh), converted and compiled from the reference's sources where they are present.
 # The sources and the converted text stay in a temporary directory outside the repository;
-# only the executables are kept (tests/programs/_build/npb_is_S, npb_dt_S), so GPU tests
+# only the executables are kept (tests/programs/_build/npb_is_{S,W,A,B,C}, npb_dt_S), so GPU tests
 # can run them and parse the benchmarks' own verification lines.
 NPB = "/root/reference/toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI"
 NPB_COMMON_DEFS = ('#define COMPILETIME "now"\n#define NPBVERSION "3.4.1"\n#define MPICC "gpumpicc"\n'
@@ -112,6 +117,14 @@ NPB_PROGRAMS = [
     # (output, benchmark dir, sources in it, common sources, npbparams.h)
     ("npb_is_S", "IS", ["is.c"], ["c_print_results.c", "c_timers.c"],
      "#define CLASS 'S'\n#define NUM_PROCS 4\n"),
+    ("npb_is_W", "IS", ["is.c"], ["c_print_results.c", "c_timers.c"],
+     "#define CLASS 'W'\n#define NUM_PROCS 16\n"),
+    ("npb_is_A", "IS", ["is.c"], ["c_print_results.c", "c_timers.c"],
+     "#define CLASS 'A'\n#define NUM_PROCS 64\n"),
+    ("npb_is_B", "IS", ["is.c"], ["c_print_results.c", "c_timers.c"],
+     "#define CLASS 'B'\n#define NUM_PROCS 64\n"),
+    ("npb_is_C", "IS", ["is.c"], ["c_print_results.c", "c_timers.c"],
+     "#define CLASS 'C'\n#define NUM_PROCS 64\n"),
     # class S parameters of NPB's setparams.c (write_dt_info)
     ("npb_dt_S", "DT", ["dt.c", "DGraph.c"], ["c_print_results.c", "c_timers.c", "randdp.c"],
      "#define CLASS 'S'\n#define NUM_PROCS 5\n#define NUM_SAMPLES 1728\n"
@@ -119,40 +132,47 @@ NPB_PROGRAMS = [
 ]
 
 
+def _build_one_npb(item, verbose):
+    import gpumpicc
+    name, bdir, srcs, common, params = item
+    out_dir = os.path.join(ROOT, "tests", "programs", "_build")
+    tmp = tempfile.mkdtemp(prefix="gpumpi_npb_")
+    try:
+        proj = os.path.join(tmp, "npb")
+        os.makedirs(os.path.join(proj, bdir))
+        os.makedirs(os.path.join(proj, "common"))
+        for f in srcs + (["DGraph.h"] if bdir == "DT" else []):
+            shutil.copy(os.path.join(NPB, bdir, f), os.path.join(proj, bdir, f))
+        for f in common + ["c_timers.h"]:
+            shutil.copy(os.path.join(NPB, "common", f), os.path.join(proj, "common", f))
+        with open(os.path.join(proj, bdir, "npbparams.h"), "w") as fh:
+            fh.write(params + NPB_COMMON_DEFS)
+        files = [os.path.join(proj, bdir, f) for f in srcs] + \
+                [os.path.join(proj, "common", f) for f in common]
+        exe = gpumpicc.build(files, os.path.join(tmp, name), project=proj,
+                             build_dir=os.path.join(tmp, name + ".gpumpi"), ranks=64)
+        final = os.path.join(out_dir, name)
+        shutil.copy(exe, final)
+        if verbose:
+            print(f"built {final}", flush=True)
+        return final
+    except Exception as e:  # a converter gap is reported, the rest of the build goes on
+        sys.stderr.write(f"NPB {name}: {e}\n")
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def build_npb(verbose=False):
+    """The NPB executables, compiled four at a time (each is one large translation unit)."""
     if not os.path.isdir(NPB):
         return []
     sys.path.insert(0, HERE)
-    import gpumpicc
-    out_dir = os.path.join(ROOT, "tests", "programs", "_build")
-    os.makedirs(out_dir, exist_ok=True)
-    outs = []
-    for name, bdir, srcs, common, params in NPB_PROGRAMS:
-        tmp = tempfile.mkdtemp(prefix="gpumpi_npb_")
-        try:
-            proj = os.path.join(tmp, "npb")
-            os.makedirs(os.path.join(proj, bdir))
-            os.makedirs(os.path.join(proj, "common"))
-            for f in srcs + (["DGraph.h"] if bdir == "DT" else []):
-                shutil.copy(os.path.join(NPB, bdir, f), os.path.join(proj, bdir, f))
-            for f in common + ["c_timers.h"]:
-                shutil.copy(os.path.join(NPB, "common", f), os.path.join(proj, "common", f))
-            with open(os.path.join(proj, bdir, "npbparams.h"), "w") as fh:
-                fh.write(params + NPB_COMMON_DEFS)
-            files = [os.path.join(proj, bdir, f) for f in srcs] + \
-                    [os.path.join(proj, "common", f) for f in common]
-            exe = gpumpicc.build(files, os.path.join(tmp, name), project=proj,
-                                 build_dir=os.path.join(tmp, name + ".gpumpi"), ranks=64)
-            final = os.path.join(out_dir, name)
-            shutil.copy(exe, final)
-            if verbose:
-                print(f"built {final}", flush=True)
-            outs.append(final)
-        except Exception as e:  # a converter gap is reported, the rest of the build goes on
-            sys.stderr.write(f"NPB {name}: {e}\n")
-        finally:
-            shutil.rmtree(tmp, ignore_errors=True)
-    return outs
+    os.makedirs(os.path.join(ROOT, "tests", "programs", "_build"), exist_ok=True)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        outs = list(pool.map(lambda it: _build_one_npb(it, verbose), NPB_PROGRAMS))
+    return [o for o in outs if o]
 
 
 def build(verbose=False):
@@ -169,8 +189,12 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed to build libgpumpi.so")
     os.replace(tmp, LIB)
     build_programs(verbose)
-    build_c_programs(verbose)
-    build_npb(verbose)
+    # the converted C programs and the NPB executables: one pool, eight compiles at a time
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=2) as pool:
+        jobs = [pool.submit(build_c_programs, verbose), pool.submit(build_npb, verbose)]
+        for j in jobs:
+            j.result()
     return LIB
 
 
