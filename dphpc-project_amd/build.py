@@ -1,7 +1,9 @@
 """Build recipe of libgpumpi.so (gfx950) -- compiled in-tree with hipcc.
 
-The device runtime is header-only and is compiled together with the built-in rank
-programs and the host C ABI as one translation unit (csrc/gpumpi.hip).  Flags:
+The device runtime is header-only.  The library is nine translation units compiled in
+parallel and linked (csrc/units.h): the host C ABI with the batched steps' host code
+(gpumpi.hip), three units of rank programs, and the many-instance batched Alltoallv (one
+unit per lines-per-lane value) and reduction kernels.  Flags:
   -ffp-contract=off          every FP expression rounds as written (bit-exact FP64
                              reductions and Monte-Carlo hit tests against the oracle)
   -packed-fp32-ops disabled  no v_pk_*_f32 instructions are generated
@@ -14,7 +16,10 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "gpumpi.hip")
+UNITS = [os.path.join(HERE, "csrc", u) for u in
+         ("gpumpi.hip", "programs_tests.hip", "programs_work.hip", "programs_heavy.hip",
+          "kernels_a2av.hip", "kernels_a2av_u2.hip", "kernels_a2av_u4.hip", "kernels_a2av_u8.hip",
+          "kernels_reduce.hip")]
 LIB = os.path.join(HERE, "libgpumpi.so")
 ARCH = os.environ.get("GPUMPI_ARCH", "gfx950")
 
@@ -26,13 +31,17 @@ def hipcc():
     raise RuntimeError("hipcc not found: the gpumpi HIP library cannot be built")
 
 
-def compile_cmd(out):
+def unit_cmd(src, obj):
     return [
-        hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+        hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
         "-ffp-contract=off", "-Wno-unused-value",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-o", out, SRC,
+        "-o", obj, src,
     ]
+
+
+def link_cmd(objs, out):
+    return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs]
 
 
 # Programs in the converter's output style, compiled as executables against the runtime
@@ -175,22 +184,37 @@ def build_npb(verbose=False):
     return [o for o in outs if o]
 
 
-def build(verbose=False):
-    """Compile libgpumpi.so (always recompiles); atomically replaces the old library."""
-    fd, tmp = tempfile.mkstemp(prefix="libgpumpi.", suffix=".so.tmp", dir=HERE)
-    os.close(fd)
-    cmd = compile_cmd(tmp)
+def _run(cmd, verbose, what):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if res.returncode != 0:
-        os.unlink(tmp)
         sys.stderr.write(res.stdout)
-        raise RuntimeError("hipcc failed to build libgpumpi.so")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"hipcc failed to build {what}")
+
+
+def build(verbose=False):
+    """Compile libgpumpi.so (always recompiles: the units in parallel, then one link) and
+    atomically replace the old library; then the test programs."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = tempfile.mkdtemp(prefix="gpumpi_units_")
+    try:
+        objs = [os.path.join(objdir, os.path.splitext(os.path.basename(u))[0] + ".o") for u in UNITS]
+        with ThreadPoolExecutor(max_workers=8) as pool:
+            for f in [pool.submit(_run, unit_cmd(u, o), verbose, os.path.basename(u)) for u, o in zip(UNITS, objs)]:
+                f.result()
+        fd, tmp = tempfile.mkstemp(prefix="libgpumpi.", suffix=".so.tmp", dir=HERE)
+        os.close(fd)
+        try:
+            _run(link_cmd(objs, tmp), verbose, "libgpumpi.so")
+        except Exception:
+            os.unlink(tmp)
+            raise
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
     build_programs(verbose)
     # the converted C programs and the NPB executables: one pool, eight compiles at a time
-    from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=2) as pool:
         jobs = [pool.submit(build_c_programs, verbose), pool.submit(build_npb, verbose)]
         for j in jobs:

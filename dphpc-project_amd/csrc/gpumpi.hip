@@ -1,18 +1,18 @@
 // libgpumpi.so -- the gfx950 device MPI runtime, its built-in rank programs and the host
-// C ABI of include/gpumpi/gpumpi.h, compiled as one translation unit (the device runtime
-// is header-only and is inlined into every rank program).
+// C ABI of include/gpumpi/gpumpi.h.  This unit holds the host API and the batched steps'
+// host code; the rank programs and the many-instance batched kernels are compiled as units
+// of their own (units.h).  The device runtime is header-only and is inlined into every
+// rank program.
 #include "runtime/mpi_api.h"
 #include "runtime/rank_kernel.h"
 #include "runtime/batched.h"
 #include "runtime/batched_reduce.h"
 #include "runtime/bucket.h"
 #include "runtime/particle_step.h"
-#include "programs/reference_tests.h"
 #include "programs/workloads.h"
-#include "programs/bucket_sort.h"
 #include "programs/serve.h"
-#include "programs/runtime_tests.h"
 #include "../../include/gpumpi/gpumpi.h"
+#include "units.h"
 
 #include <algorithm>
 #include <atomic>
@@ -134,41 +134,18 @@ __global__ void heap_reset_kernel(Runtime* R, int nstates) {
     }
 }
 
-struct ProgramEntry {
-    const char* name;
-    const void* kernel;
-};
-
-#define GPUMPI_PROGRAM(name, fn) {name, (const void*)&rank_grid_kernel<fn>}
-static const ProgramEntry kPrograms[] = {
-    GPUMPI_PROGRAM("t_single_int", progs::t_single_int),
-    GPUMPI_PROGRAM("t_array", progs::t_array),
-    GPUMPI_PROGRAM("t_sendrecv", progs::t_sendrecv),
-    GPUMPI_PROGRAM("t_repeat", progs::t_repeat),
-    GPUMPI_PROGRAM("t_flood", progs::t_flood),
-    GPUMPI_PROGRAM("t_p2p_all_to_all", progs::t_p2p_all_to_all),
-    GPUMPI_PROGRAM("t_reduce_prod_int16", progs::t_reduce_prod_int16),
-    GPUMPI_PROGRAM("t_allreduce_sum_double", progs::t_allreduce_sum_double),
-    GPUMPI_PROGRAM("t_allgather", progs::t_allgather),
-    GPUMPI_PROGRAM("t_scatter", progs::t_scatter),
-    GPUMPI_PROGRAM("t_scatterv", progs::t_scatterv),
-    GPUMPI_PROGRAM("t_comm_rank", progs::t_comm_rank),
-    GPUMPI_PROGRAM("t_comm_dup", progs::t_comm_dup),
-    GPUMPI_PROGRAM("t_comm_split", progs::t_comm_split),
-    GPUMPI_PROGRAM("t_op_table", progs::t_op_table),
-    GPUMPI_PROGRAM("t_lds_buffers", progs::t_lds_buffers),
-    GPUMPI_PROGRAM("mcpi", progs::w_mcpi),
-    GPUMPI_PROGRAM("p2p", progs::w_p2p),
-    GPUMPI_PROGRAM("coll", progs::w_coll),
-    GPUMPI_PROGRAM("churn", progs::w_churn),
-    GPUMPI_PROGRAM("exchange", progs::w_exchange),
-    GPUMPI_PROGRAM("bucket_sort", progs::w_bucket_sort),
-    GPUMPI_PROGRAM("particles", progs::w_particles),
-    GPUMPI_PROGRAM("job", progs::w_job),
-    GPUMPI_PROGRAM("serve", progs::w_serve),
-};
-#undef GPUMPI_PROGRAM
-constexpr int kNumPrograms = (int)(sizeof(kPrograms) / sizeof(kPrograms[0]));
+// The built-in rank programs: tables from programs_tests.hip, programs_work.hip and
+// programs_heavy.hip, joined in that order on first use.
+static const std::vector<ProgramEntry>& programs() {
+    static const std::vector<ProgramEntry> all = [] {
+        std::vector<ProgramEntry> v;
+        v.insert(v.end(), kProgramsTests, kProgramsTests + kNumProgramsTests);
+        v.insert(v.end(), kProgramsWork, kProgramsWork + kNumProgramsWork);
+        v.insert(v.end(), kProgramsHeavy, kProgramsHeavy + kNumProgramsHeavy);
+        return v;
+    }();
+    return all;
+}
 
 }  // namespace gpumpi
 
@@ -414,17 +391,17 @@ const char* gpumpi_last_error(gpumpi_runtime* rt) {
     return rt ? rt->err.c_str() : g_host_error.c_str();
 }
 
-int gpumpi_program_count(void) { return kNumPrograms; }
+int gpumpi_program_count(void) { return (int)programs().size(); }
 
 uint64_t gpumpi_layout_stamp(void) { return kLayoutStamp; }
 const char* gpumpi_program_name(int i) {
-    return (i >= 0 && i < kNumPrograms) ? kPrograms[i].name : nullptr;
+    return (i >= 0 && i < (int)programs().size()) ? programs()[i].name : nullptr;
 }
 
 static const ProgramEntry* find_program(const char* name) {
     if (!name) return nullptr;
-    for (int i = 0; i < kNumPrograms; i++)
-        if (std::strcmp(kPrograms[i].name, name) == 0) return &kPrograms[i];
+    for (const ProgramEntry& e : programs())
+        if (std::strcmp(e.name, name) == 0) return &e;
     return nullptr;
 }
 
@@ -861,18 +838,9 @@ struct gpumpi_a2av_plan {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-template <bool DEFER, bool FUSE>
-static const void* a2av_copy_for_mode(int U, bool nt) {
-    switch (U) {
-    case 2: return nt ? (const void*)&a2av_copy_kernel<2, true, DEFER, FUSE> : (const void*)&a2av_copy_kernel<2, false, DEFER, FUSE>;
-    case 4: return nt ? (const void*)&a2av_copy_kernel<4, true, DEFER, FUSE> : (const void*)&a2av_copy_kernel<4, false, DEFER, FUSE>;
-    default: return nt ? (const void*)&a2av_copy_kernel<8, true, DEFER, FUSE> : (const void*)&a2av_copy_kernel<8, false, DEFER, FUSE>;
-    }
-}
-
+// the copy kernel's instances are compiled in kernels_a2av.hip
 static const void* a2av_copy_for(int U, bool nt, bool defer, bool fuse) {
-    if (fuse) return defer ? a2av_copy_for_mode<true, true>(U, nt) : a2av_copy_for_mode<false, true>(U, nt);
-    return defer ? a2av_copy_for_mode<true, false>(U, nt) : a2av_copy_for_mode<false, false>(U, nt);
+    return a2av_copy_kernel_for(U, nt, defer, fuse);
 }
 
 // Copy grid: one workgroup per chunk of the last measured step (P extra for the partial
@@ -2062,7 +2030,7 @@ int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_
     if (gpumpi_device_info(&cus, nullptr, nullptr) != GPUMPI_OK) return GPUMPI_E_HIP;
     const uint64_t bytes = es > 0 ? (uint64_t)count * (uint64_t)es : 0;
     const bool wide = bytes / (1024ull * kReduceWideU) >= 4ull * (uint64_t)cus;
-    const void* k = es > 0 ? batched_reduce_kernel_for(datatype, op, wide) : nullptr;
+    const void* k = es > 0 ? batched_reduce_kernel_lookup(datatype, op, wide) : nullptr;
     if (!k) {
         g_host_error = "batched reduction: the pair (datatype " + std::to_string(datatype) +
                        ", operator " + std::to_string(op) + ") is not on the batched path "

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "../runtime/mpi_api.h"
+#include "workloads.h"
 
 namespace gpumpi {
 namespace progs {

@@ -167,7 +167,7 @@ __device__ __forceinline__ void a2av_plan_row(const A2avDev& p, int64_t d, int64
     }
 }
 
-__global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
+static __global__ void __launch_bounds__(256) a2av_plan_kernel(A2avDev p) {
     __shared__ int64_t s_tot[4];
     a2av_plan_row(p, blockIdx.x, s_tot);
 }
@@ -214,7 +214,7 @@ __device__ __forceinline__ void a2av_scan_rows(const A2avDev& p, uint64_t* part,
     }
 }
 
-__global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
+static __global__ void __launch_bounds__(1024) a2av_scan_kernel(A2avDev p) {
     __shared__ uint64_t part[16];
     __shared__ int64_t bpart[16];
     a2av_scan_rows<1024>(p, part, bpart);

@@ -45,7 +45,7 @@ __device__ __forceinline__ uint64_t p2p_chunks(const char* dst, uint64_t bytes) 
 
 // chunk_off[i] = chunks of pairs 0 .. i-1, for i = 0 .. n (chunk_off[n] = all chunks).
 // Passes of 1024 pairs, one per thread: wave scans, wave totals through LDS, a carry.
-__global__ __launch_bounds__(kP2pScanThreads) void p2p_scan_kernel(const P2pPair* pairs, int64_t n,
+static __global__ __launch_bounds__(kP2pScanThreads) void p2p_scan_kernel(const P2pPair* pairs, int64_t n,
                                                                    int64_t* chunk_off) {
     constexpr int kWaves = kP2pScanThreads / kWave;
     __shared__ int64_t s_w[kWaves];
@@ -112,7 +112,7 @@ __device__ __forceinline__ void p2p_copy_piece(char* dst, const char* src, uint6
     }
 }
 
-__global__ __launch_bounds__(kP2pThreads) void p2p_copy_kernel(const P2pPair* pairs, int64_t n,
+static __global__ __launch_bounds__(kP2pThreads) void p2p_copy_kernel(const P2pPair* pairs, int64_t n,
                                                                const int64_t* chunk_off) {
     if (n <= 0) return;
     const uint64_t total = (uint64_t)chunk_off[n];

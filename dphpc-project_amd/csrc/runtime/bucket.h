@@ -136,7 +136,7 @@ __host__ __device__ __forceinline__ uint32_t npb_key(int r, int P, int64_t total
     return (uint32_t)(s >> (48 - max_key_log2));
 }
 
-__global__ void gen_keys_kernel(uint32_t* keys, int P, int64_t n, uint64_t seed, uint32_t step,
+static __global__ void gen_keys_kernel(uint32_t* keys, int P, int64_t n, uint64_t seed, uint32_t step,
                                 int dist, int key_bits, const double* cdf, int64_t ncdf) {
     const int64_t total = (int64_t)P * n;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -149,7 +149,7 @@ __global__ void gen_keys_kernel(uint32_t* keys, int P, int64_t n, uint64_t seed,
 }
 
 // grid: (chunks per rank, P); hist: [P][nb]
-__global__ void bucket_hist_kernel(const uint32_t* keys, int64_t n, int shift, int nb,
+static __global__ void bucket_hist_kernel(const uint32_t* keys, int64_t n, int shift, int nb,
                                    uint32_t* hist) {
     extern __shared__ uint32_t lh[];
     const int r = blockIdx.y;
@@ -165,7 +165,7 @@ __global__ void bucket_hist_kernel(const uint32_t* keys, int64_t n, int shift, i
 }
 
 // totals[b] = sum over ranks of hist[r][b]
-__global__ void bucket_totals_kernel(const uint32_t* hist, int P, int nb, int64_t* totals) {
+static __global__ void bucket_totals_kernel(const uint32_t* hist, int P, int nb, int64_t* totals) {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         int64_t s = 0;
         for (int r = 0; r < P; r++) s += hist[(int64_t)r * nb + b];
@@ -174,7 +174,7 @@ __global__ void bucket_totals_kernel(const uint32_t* hist, int P, int nb, int64_
 }
 
 // NPB splitters (is.c:721-751), one thread: bucket range [lo[j], hi[j]] of rank j.
-__global__ void bucket_splitters_kernel(const int64_t* totals, int P, int nb, int64_t n,
+static __global__ void bucket_splitters_kernel(const int64_t* totals, int P, int nb, int64_t n,
                                         int32_t* lo, int32_t* hi) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     int64_t acc = 0;
@@ -197,7 +197,7 @@ __global__ void bucket_splitters_kernel(const int64_t* totals, int P, int nb, in
 
 // One wavefront per source rank: send_counts[r][d] = sum of hist[r][lo_d..hi_d],
 // send_displs = exclusive prefix over d.
-__global__ void bucket_counts_kernel(const uint32_t* hist, int P, int nb, const int32_t* lo,
+static __global__ void bucket_counts_kernel(const uint32_t* hist, int P, int nb, const int32_t* lo,
                                      const int32_t* hi, int32_t* scounts, int32_t* sdispls) {
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P) return;
@@ -218,7 +218,7 @@ __global__ void bucket_counts_kernel(const uint32_t* hist, int P, int nb, const 
 }
 
 // Stable bucketing: 4 ranks per 256-thread block, LDS bucket pointers per wave.
-__global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* keys, int P, int64_t n,
+static __global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* keys, int P, int64_t n,
                                                             int shift, int nb, int bits,
                                                             const uint32_t* hist, uint32_t* out) {
     extern __shared__ uint32_t sptr[];
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(256) bucket_scatter_kernel(const uint32_t* key
 // rank d's sorted keys are the slice [recv_off_d, recv_off_d + recv_tot_d) of the output.
 constexpr int kSortLdsBins = 16384;  // keys below this are counted in LDS first
 
-__global__ void __launch_bounds__(1024) sort_hist_kernel(const uint32_t* keys, int64_t n,
+static __global__ void __launch_bounds__(1024) sort_hist_kernel(const uint32_t* keys, int64_t n,
                                                          uint32_t nbins, uint32_t* hist) {
     __shared__ uint32_t lh[kSortLdsBins];
     const uint32_t lds_bins = nbins < kSortLdsBins ? nbins : kSortLdsBins;
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(1024) sort_hist_kernel(const uint32_t* keys, i
 }
 
 // Exclusive scan of nbins counters in three kernels (4096 counters per 1024-thread block).
-__global__ void __launch_bounds__(1024) scan_block_sums_kernel(const uint32_t* h, uint32_t nbins,
+static __global__ void __launch_bounds__(1024) scan_block_sums_kernel(const uint32_t* h, uint32_t nbins,
                                                                uint64_t* sums) {
     __shared__ uint64_t red[16];
     const uint64_t base = (uint64_t)blockIdx.x * 4096;
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(1024) scan_block_sums_kernel(const uint32_t* h
     }
 }
 
-__global__ void __launch_bounds__(1024) scan_sums_kernel(uint64_t* sums, int nblocks) {
+static __global__ void __launch_bounds__(1024) scan_sums_kernel(uint64_t* sums, int nblocks) {
     // single block, exclusive scan in place (sequential chunks of 1024)
     __shared__ uint64_t part[16];
     uint64_t carry = 0;
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(1024) scan_sums_kernel(uint64_t* sums, int nbl
     }
 }
 
-__global__ void __launch_bounds__(1024) scan_apply_kernel(const uint32_t* h, uint32_t nbins,
+static __global__ void __launch_bounds__(1024) scan_apply_kernel(const uint32_t* h, uint32_t nbins,
                                                           const uint64_t* sums, uint32_t* prefix) {
     __shared__ uint64_t part[16];
     const uint64_t base = (uint64_t)blockIdx.x * 4096;
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(1024) scan_apply_kernel(const uint32_t* h, uin
 }
 
 // Writes each value's run; runs longer than 256 go to a list filled by whole blocks.
-__global__ void sort_expand_kernel(const uint32_t* hist, const uint32_t* prefix, uint32_t nbins,
+static __global__ void sort_expand_kernel(const uint32_t* hist, const uint32_t* prefix, uint32_t nbins,
                                    uint32_t* out, uint32_t* big, uint32_t* nbig) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nbins;
          v += (uint64_t)gridDim.x * blockDim.x) {
@@ -365,7 +365,7 @@ __global__ void sort_expand_kernel(const uint32_t* hist, const uint32_t* prefix,
     }
 }
 
-__global__ void sort_fill_runs_kernel(const uint32_t* hist, const uint32_t* prefix,
+static __global__ void sort_fill_runs_kernel(const uint32_t* hist, const uint32_t* prefix,
                                       const uint32_t* big, const uint32_t* nbig, uint32_t* out) {
     const uint32_t nb = *nbig;
     for (uint32_t r = blockIdx.x; r < nb; r += gridDim.x) {
@@ -376,7 +376,7 @@ __global__ void sort_fill_runs_kernel(const uint32_t* hist, const uint32_t* pref
 }
 
 // Number of keys < bound in keys[0, n) (partial verification of NPB IS).
-__global__ void count_less_kernel(const uint32_t* keys, int64_t n, uint32_t bound,
+static __global__ void count_less_kernel(const uint32_t* keys, int64_t n, uint32_t bound,
                                   unsigned long long* out) {
     uint64_t c = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

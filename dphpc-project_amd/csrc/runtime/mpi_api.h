@@ -6,7 +6,9 @@
 #include "collectives.h"
 #include "../../../include/gpumpi/gpu_malloc.h"
 
-__device__ gpumpi::OpImpl gpumpi_builtin_ops[12] = {
+// Weak: each translation unit that includes the runtime is its own code object and needs
+// its own copy; the host-side shadow symbols of the copies are merged at link time.
+__device__ __attribute__((weak)) gpumpi::OpImpl gpumpi_builtin_ops[12] = {
     {0, nullptr, 1}, {1, nullptr, 1}, {2, nullptr, 1}, {3, nullptr, 1},
     {4, nullptr, 1}, {5, nullptr, 1}, {6, nullptr, 1}, {7, nullptr, 1},
     {8, nullptr, 1}, {9, nullptr, 1}, {10, nullptr, 1}, {11, nullptr, 1}};

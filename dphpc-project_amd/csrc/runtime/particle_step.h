@@ -140,7 +140,7 @@ __device__ __forceinline__ int32_t ps_wave_scan(int32_t v) {
 
 // Particles of the workload's initial state: n0 per rank, positions uniform in the rank's
 // cell on a 2^-20 lattice, velocities on a 2^-15 lattice, id = rank * n0 + k.
-__global__ __launch_bounds__(kPsThreads) void ps_init_kernel(int gx, int gy, int gz, int64_t cap,
+static __global__ __launch_bounds__(kPsThreads) void ps_init_kernel(int gx, int gy, int gz, int64_t cap,
                                                              int64_t n0, uint64_t seed, double* parts,
                                                              int64_t* counts) {
     const int P = gx * gy * gz;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kPsThreads) void ps_init_kernel(int gx, int gy, int
 }
 
 // Direction bytes of (in, cin) at `step`; resets the step counter and the statistics.
-__global__ __launch_bounds__(kPsThreads) void ps_dirs_kernel(PartStepDev a, uint32_t step) {
+static __global__ __launch_bounds__(kPsThreads) void ps_dirs_kernel(PartStepDev a, uint32_t step) {
     const int64_t i = (int64_t)blockIdx.x * kPsThreads + threadIdx.x;
     if (i == 0) {
         a.step[0] = step;
@@ -187,7 +187,7 @@ typedef u32x4 ps_u32x4_a1 __attribute__((aligned(1)));   // a slab may start at 
 //         the 16 threads of a chunk sum their words and store the chunk's counts;
 //  scan:  after a barrier, thread t takes chunk c0 + t of each pass of 256 chunks: wave
 //         scans, wave totals through LDS, a running carry between passes.
-__global__ __launch_bounds__(kPsTallyThreads) void ps_tally_kernel(PartStepDev a) {
+static __global__ __launch_bounds__(kPsTallyThreads) void ps_tally_kernel(PartStepDev a) {
     const int r = blockIdx.x;
     const int t = threadIdx.x, wv = t >> 6, l = t & 63;
     if (r == 0 && t == 0) a.stats[1] = 0;

@@ -22,23 +22,33 @@ LIB = os.path.join(PKG_DIR, "libgpumpi.so")
 HEAVY = {"w_bucket_sort", "w_particles"}
 
 
+MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+
+
 def _kernels(tmp_path):
+    """Kernel name -> VGPRs + AGPRs per lane, over the code objects of every translation
+    unit (the library's .hip_fatbin section holds one offload bundle per unit)."""
     tools = [os.path.join(LLVM, t) for t in ("llvm-objcopy", "clang-offload-bundler", "llvm-readelf")]
     if not all(os.path.exists(t) for t in tools) or not os.path.exists(LIB):
         pytest.skip("LLVM tools or the built library missing")
     fat = tmp_path / "fatbin.bin"
-    co = tmp_path / "gfx950.co"
     subprocess.run([tools[0], "-O", "binary", "--only-section=.hip_fatbin", LIB, str(fat)], check=True)
-    subprocess.run([tools[1], "--unbundle", "--type=o", f"--input={fat}",
-                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
-    notes = subprocess.run([tools[2], "--notes", str(co)], capture_output=True, text=True,
-                           check=True).stdout
+    data = fat.read_bytes()
+    starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
     out = {}
-    for item in re.split(r"\n  - ", notes):
-        m = re.search(r"\.name:\s+(\S+)", item)
-        v = re.search(r"\.vgpr_count:\s+(\d+)", item)
-        if m and v:
-            out[m.group(1)] = int(v.group(1))
+    for i, a in enumerate(starts):
+        piece = tmp_path / f"bundle{i}.bin"
+        piece.write_bytes(data[a:starts[i + 1] if i + 1 < len(starts) else len(data)])
+        co = tmp_path / f"gfx950_{i}.co"
+        subprocess.run([tools[1], "--unbundle", "--type=o", f"--input={piece}",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"], check=True)
+        notes = subprocess.run([tools[2], "--notes", str(co)], capture_output=True, text=True,
+                               check=True).stdout
+        for item in re.split(r"\n  - ", notes):
+            m = re.search(r"\.name:\s+(\S+)", item)
+            v = re.search(r"\.vgpr_count:\s+(\d+)", item)
+            if m and v:
+                out[m.group(1)] = max(out.get(m.group(1), 0), int(v.group(1)))
     return out
 
 
