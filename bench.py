@@ -466,6 +466,36 @@ def p2p_sweep(pkg, torch, args):
     return out
 
 
+def coll_profile(pkg, torch, args):
+    """The rank grid's large collectives for a counter run: MPI_INT and MPI_DOUBLE, 2^22
+    elements per rank, 1024 ranks.  Each kind runs twice: a launch that fills the inputs
+    (Philox) and makes one call, then a launch with no_fill = 1 that makes exactly one call,
+    so that launch's FETCH_SIZE / WRITE_SIZE are one collective's traffic.  Algorithmic
+    bytes per call: Bcast P*N*s written (+ N*s read), Reduce P*N*s read (+ N*s written),
+    Allreduce P*N*s read + P*N*s written."""
+    from dphpc_project_amd import programs as pr
+    P, n = 1024, 1 << 22
+    inb = torch.zeros(P * n * 8 + 16, dtype=torch.uint8, device="cuda")
+    outb = torch.zeros_like(inb)
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    if args.copy_probe:
+        a = torch.empty(1 << 27, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    with pkg.Runtime(P, heap_bytes=128 << 20, timeout_s=60.0) as rt:
+        for dtype, dname, es in ((2, "MPI_INT", 4), (13, "MPI_DOUBLE", 8)):
+            for kind in ("bcast", "reduce", "allreduce"):
+                a = pr.CollArgs(0x1234, pr.COLL_KINDS[kind], dtype, 0, 0, n, 1, 0, inb.data_ptr(),
+                                outb.data_ptr(), n * es, n * es, sec.data_ptr())
+                rt.run("coll", a)
+                a.no_fill = 1
+                rt.run("coll", a)
+                torch.cuda.synchronize()
+                log(f"coll profile {dname} {kind}: one call {float(sec.max().item()) * 1e6:.0f} us")
+
+
 def coll_sweep(pkg, torch, args):
     """Collectives on 1024 wavefront ranks: Bcast, Reduce and Allreduce (MPI_SUM and
     MPI_MAX) and Alltoall, on MPI_INT (uniform in [-2^20, 2^20]) and MPI_DOUBLE (N(0,1)),
@@ -783,6 +813,10 @@ def main():
                     help="block_threads,blocks_per_cu,lines_per_lane,nontemporal[,chunk_rounds[,defer_straddle]]")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (scale for rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--coll-profile", action="store_true",
+                    help="only the rank-grid Bcast, Reduce and Allreduce at 2^22 elements on "
+                         "1024 ranks, one call per launch (for rocprofv3 kernel-trace and "
+                         "FETCH/WRITE_SIZE runs), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU: it measures the "
@@ -798,6 +832,9 @@ def main():
     P, n = args.ranks, 1 << args.keys_log2
     stream = torch.cuda.current_stream().cuda_stream
     log(f"device: {torch.cuda.get_device_name(0)}; info {pkg.device_info()}")
+    if args.coll_profile:
+        coll_profile(pkg, torch, args)
+        return
 
     # ---------------- batched path, uniform keys
     ex = Exchange(pkg, torch, P, n, "uniform")
