@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
     const int j = (int)((int64_t)blockIdx.x * kPsThreads + threadIdx.x);
     if (j == 0) a.step[0] += 1u;
     if (j >= a.P) return;
-    // round 1: j's neighbours (its possible sources) and its stayers
+    // step 1: j's neighbours (its possible sources) and its stayers
     const int4* nt = (const int4*)a.nbr_tab;
     const int4 n0 = nt[2 * j], n1 = nt[2 * j + 1];
     const int64_t stay = a.tot[(int64_t)j * kPsClasses + kPsStay];
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
             src[i + 1] = hi;
         }
     }
-    // round 2: the class of each source toward j (the smallest direction reaching j)
+    // step 2: the class of each source toward j (the smallest direction reaching j)
     int cl[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kPsThreads) void ps_place_kernel(PartStepDev a) {
         for (int d = 5; d >= 0; d--) c = m[d] == j ? d : c;
         cl[i] = c;
     }
-    // round 3: the sources' counts toward j
+    // step 3: the sources' counts toward j
     int64_t cnt[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) cnt[i] = a.tot[(int64_t)src[i] * kPsClasses + cl[i]];

@@ -9,7 +9,7 @@
  *     (P ulps of the total; a runtime that folds in rank order gives the same bits, and
  *     both values are printed so a test can check that);
  *   - the total is pi within the tolerance.
- * The verdict is broadcast, and every rank exits with status 0 only if both hold. */
+ * The result is broadcast, and every rank exits with status 0 only if both hold. */
 #include <stdio.h>
 #include <stdlib.h>
 #include <math.h>
@@ -50,18 +50,18 @@ int main(int argc, char **argv) {
     if (rank == 0) blocks = (double *)malloc(sizeof(double) * (size_t)size);
     MPI_Gather(&block, 1, MPI_DOUBLE, blocks, 1, MPI_DOUBLE, 0, MPI_COMM_WORLD);
 
-    int verdict = 1;
+    int passed = 1;
     if (rank == 0) {
         double sum = blocks[0];
         for (int r = 1; r < size; r++) sum = blocks[r] + sum;
         const double err = fabs(total - pi);
-        verdict = fabs(sum - total) <= size * 2.3e-16 * total && err < setup[1];
+        passed = fabs(sum - total) <= size * 2.3e-16 * total && err < setup[1];
         printf("quad: %d blocks x %ld panels, integral %.15f, |error| %.3e, %s\n", size,
-               (long)setup[0], total, err, verdict ? "ok" : "FAILED");
+               (long)setup[0], total, err, passed ? "ok" : "FAILED");
         printf("quad: reduced %.17g gathered %.17g\n", total, sum);
         free(blocks);
     }
-    MPI_Bcast(&verdict, 1, MPI_INT, 0, MPI_COMM_WORLD);
+    MPI_Bcast(&passed, 1, MPI_INT, 0, MPI_COMM_WORLD);
     MPI_Finalize();
-    return verdict ? 0 : 1;
+    return passed ? 0 : 1;
 }
