@@ -76,7 +76,23 @@ struct P2pArgs {
 };
 
 __device__ void fill_payload(uint8_t* p, uint64_t n, uint64_t seed, int rank) {
-    for (uint64_t b = (uint64_t)lane(); b * 16 < n; b += kWave) {
+    // bulk (16-byte aligned buffers): 4 independent Philox blocks per lane and pass, so the
+    // multiplies of one block's rounds overlap the others' (one wave: 5.9 -> 7.9 GB/s)
+    constexpr int kI = 4;
+    uint64_t b0 = 0;
+    if (((uintptr_t)p & 15) == 0) {
+        const uint64_t nb = n / 16, full = nb - nb % ((uint64_t)kWave * kI);
+        for (; b0 < full; b0 += (uint64_t)kWave * kI) {
+            u32x4v v[kI];
+#pragma unroll
+            for (int i = 0; i < kI; i++)
+                v[i] = rank_stream(seed, rank, 0u, b0 + (uint64_t)i * kWave + lane(), kStreamBytes);
+#pragma unroll
+            for (int i = 0; i < kI; i++)
+                *(u32x4*)(p + 16 * (b0 + (uint64_t)i * kWave + lane())) = u32x4{v[i].x, v[i].y, v[i].z, v[i].w};
+        }
+    }
+    for (uint64_t b = b0 + (uint64_t)lane(); b * 16 < n; b += kWave) {
         u32x4v v = rank_stream(seed, rank, 0u, b, kStreamBytes);
         uint64_t o = b * 16;
         if (o + 16 <= n && ((uintptr_t)(p + o) & 15) == 0) {
