@@ -459,6 +459,7 @@ def p2p_sweep(pkg, torch, args):
                 us.append(round(one_way * 1e6, 3))
                 gbs.append(round(msgs * b / one_way / 1e9, 2))
             out[name] = {"us": us, "GBps": gbs}
+            out["copy_stage_bytes"] = rt.copy_stage_bytes()
             log(f"p2p {name}: 8 B {us[0]:.2f} us, 64 KiB {us[13]:.2f} us, 64 MiB {us[-1]:.1f} us "
                 f"({gbs[-1]:.0f} GB/s)")
     del sb, rb

@@ -40,6 +40,7 @@ class Config(ctypes.Structure):
         ("num_ranks", ctypes.c_int32), ("ranks_per_block", ctypes.c_int32),
         ("ring_slots", ctypes.c_int32), ("max_requests", ctypes.c_int32),
         ("max_unexpected", ctypes.c_int32), ("bcast_push_min", ctypes.c_int32),
+        ("copy_stage_kib", ctypes.c_int32), ("reserved1", ctypes.c_int32),
         ("heap_bytes", ctypes.c_uint64), ("stack_bytes", ctypes.c_uint64),
         ("timeout_s", ctypes.c_double),
     ]
@@ -87,6 +88,7 @@ _SIGNATURES = {
     "gpumpi_main_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
     "gpumpi_layout_stamp": (ctypes.c_uint64, []),
     "gpumpi_last_error": (ctypes.c_char_p, [_P]),
+    "gpumpi_runtime_copy_stage": (ctypes.c_int, [_P]),
     "gpumpi_last_device_error": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "gpumpi_program_count": (ctypes.c_int, []),
     "gpumpi_program_name": (ctypes.c_char_p, [ctypes.c_int]),
@@ -274,6 +276,10 @@ class Runtime:
 
     def last_error(self):
         return self.lib.gpumpi_last_error(self.h).decode()
+
+    def copy_stage_bytes(self):
+        """LDS staging bytes per rank that the last launch gave (gpumpi_runtime_copy_stage)."""
+        return int(self.lib.gpumpi_runtime_copy_stage(self.h))
 
     def device_error(self):
         code, rank = _I32(), _I32()

@@ -166,6 +166,7 @@ struct gpumpi_runtime {
     std::string err;
     bool needs_reset = false;
     int world_size = 0;  // ranks of MPI_COMM_WORLD as last set on the device
+    int stage_dev = 0;   // Runtime::copy_stage_bytes as last set on the device
     IoSlot* io = nullptr;  // pinned host I/O request slots, one per rank
     class IoService* io_service = nullptr;
 };
@@ -394,6 +395,7 @@ const char* gpumpi_last_error(gpumpi_runtime* rt) {
 int gpumpi_program_count(void) { return (int)programs().size(); }
 
 uint64_t gpumpi_layout_stamp(void) { return kLayoutStamp; }
+int gpumpi_runtime_copy_stage(const gpumpi_runtime* rt) { return rt ? rt->stage_dev : 0; }
 const char* gpumpi_program_name(int i) {
     return (i >= 0 && i < (int)programs().size()) ? programs()[i].name : nullptr;
 }
@@ -432,15 +434,35 @@ static int stage_args(gpumpi_runtime* rt, const void* args, size_t bytes, hipStr
 // bounded waits report it.  Rank programs use at most 256 registers per lane where the
 // runtime alone is concerned (two waves per SIMD: 1024 ranks on 256 CUs); a program whose
 // own loops need more holds one wave per SIMD and runs up to 512 ranks.
+//
+// Each wave may also get an LDS staging area for its large copies (dynamic LDS,
+// copy_stage_kib): the largest candidate with which the grid still passes that rule.  A
+// workgroup's dynamic LDS stays within 64 KiB.
 static int launch_rank_grid(gpumpi_runtime* rt, const void* kernel, int nranks, int argc,
                             char** d_argv, hipStream_t s) {
     const int rpb = rt->cfg.ranks_per_block;
     const int block = 64 * rpb;
     const int grid = (nranks + rpb - 1) / rpb;
-    int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
-    const int64_t slots = (int64_t)per_cu * rpb * rt->cus;  // wave slots this kernel can use
     const int64_t need = (int64_t)grid * rpb;
+    const int kib = rt->cfg.copy_stage_kib;
+    int cand[3], nc = 0;
+    if (kib == 0) {
+        cand[nc++] = 32 << 10;
+        cand[nc++] = 16 << 10;
+    } else if (kib > 0 && kib <= 64) {
+        cand[nc++] = kib << 10;
+    }
+    cand[nc++] = 0;
+    int per_cu = 0, stage = 0;
+    int64_t slots = 0;
+    for (int i = 0; i < nc; i++) {
+        const size_t dyn = (size_t)cand[i] * (size_t)rpb;
+        if (dyn > (64u << 10)) continue;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, dyn));
+        slots = (int64_t)per_cu * rpb * rt->cus;  // wave slots this kernel can use
+        stage = cand[i];
+        if (need <= slots / 2) break;
+    }
     if (need > slots / 2) {
         set_err(rt, "rank grid of " + std::to_string(need) + " wavefronts exceeds half of the " +
                         std::to_string(slots) + " wave slots this program can occupy (" +
@@ -448,9 +470,15 @@ static int launch_rank_grid(gpumpi_runtime* rt, const void* kernel, int nranks, 
                         " per CU)");
         return GPUMPI_E_RESIDENCY;
     }
+    if (stage != rt->stage_dev) {
+        rt->h.copy_stage_bytes = stage;
+        HIP_TRY(hipMemcpyAsync((char*)rt->d_rt + offsetof(Runtime, copy_stage_bytes),
+                               &rt->h.copy_stage_bytes, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        rt->stage_dev = stage;
+    }
     Runtime* d = rt->d_rt;
     void* kargs[] = {(void*)&d, (void*)&argc, (void*)&d_argv};
-    HIP_TRY(hipLaunchKernel(kernel, dim3(grid), dim3(block), kargs, 0, s));
+    HIP_TRY(hipLaunchKernel(kernel, dim3(grid), dim3(block), kargs, (size_t)stage * (size_t)rpb, s));
     return GPUMPI_OK;
 }
 

@@ -10,6 +10,7 @@ namespace gpumpi {
 template <int (*MAIN)(int, char**)>
 __global__ void __launch_bounds__(256) rank_grid_kernel(Runtime* R, int argc, char** argv) {
     s_rt = R;
+    s_stage_bytes = (uint32_t)R->copy_stage_bytes;   // this launch's LDS stage per wave
     const int rank = world_rank();
     if (rank >= R->nranks) return;
     int rc = MAIN(argc, argv);

@@ -163,7 +163,7 @@ struct Runtime {
     int32_t exit_code;    // atomic or
     int32_t err_lock;     // atomic
     int32_t bcast_push_min;  // Bcast pushes slices from this many bytes per rank; < 0 never
-    int32_t pad0;
+    int32_t copy_stage_bytes;  // LDS staging area per wave of the running launch (0 = none)
     uint64_t timeout_ticks;
     double wtick;         // seconds per wall_clock64 tick
     uint64_t next_ctx;    // atomic context-id allocator

@@ -61,6 +61,12 @@ typedef struct gpumpi_config {
                                    ones are pulled by each receiver.  0 = 512 with at
                                    least 512 KiB in all (the measured crossover on 64 and
                                    1024 ranks), < 0 never */
+    int32_t copy_stage_kib;     /* LDS staging area per rank (wavefront) for its copies of
+                                   64 KiB and more: 0 = the largest of 32 and 16 KiB with
+                                   which the grid keeps its residency headroom (none if
+                                   neither does), > 0 = this many KiB if it fits, < 0 never.
+                                   The launch gives it as dynamic LDS. */
+    int32_t reserved1;          /* must be 0 */
     uint64_t heap_bytes;        /* -p: device heap arena; 0 = 1 GiB */
     uint64_t stack_bytes;       /* -s: per-lane stack limit; 0 = runtime default */
     double timeout_s;           /* bound on every device-side wait; 0 = 20 s */
@@ -84,6 +90,10 @@ int gpumpi_run(gpumpi_runtime* rt, const char* program, int argc, const char* co
 /* Launch without waiting (the caller synchronises the stream). */
 int gpumpi_run_async(gpumpi_runtime* rt, const char* program, const void* args,
                      size_t args_bytes, void* stream);
+
+/* Bytes of LDS staging per wavefront that the last launch on `rt` gave its ranks
+ * (gpumpi_config.copy_stage_kib), 0 if none or nothing was launched yet. */
+int gpumpi_runtime_copy_stage(const gpumpi_runtime* rt);
 
 /* Launch a rank-grid kernel compiled into the caller's own module (a converted program,
  * see gpumpi/gpumpi_main.h) with the same semantics as gpumpi_run. */

@@ -97,6 +97,52 @@ def test_p2p_256_ranks(mode, nbytes, pkg, oracle, torch_gpu):
         assert np.array_equal(got, fill_bytes(oracle, seed, origin, nbytes)), r
 
 
+@pytest.mark.parametrize("stage_kib", [-1, 16, 32])
+@pytest.mark.parametrize("offset,nbytes", [(0, 1 << 20), (5, 3 * 32768 + 16 * 7), (0, 65536 + 16)])
+@pytest.mark.parametrize("mode", [0, 3])
+def test_p2p_lds_staged_copies(mode, offset, nbytes, stage_kib, pkg, oracle, torch_gpu):
+    """Large messages through the LDS-staged copy (copy_stage_kib 16 / 32) and through
+    registers only (-1), byte-exact against the oracle.  Buffers at offset 5 give every
+    message an 11-byte head before the 16-byte aligned part and a byte tail after it;
+    3 x 32 KiB + 112 leaves a part batch after the whole stage batches."""
+    from dphpc_project_amd import programs as pr
+    torch = torch_gpu
+    P, seed = 16, 0x5ba
+    print(f"[gpu] p2p staged mode={mode} offset={offset} bytes={nbytes} stage={stage_kib}", flush=True)
+    sb = torch.zeros(P * nbytes + 16, dtype=torch.uint8, device="cuda")
+    rb = torch.zeros(P * nbytes + 16, dtype=torch.uint8, device="cuda")
+    sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+    mis = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+    with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=20.0, copy_stage_kib=stage_kib) as rt:
+        rt.run("p2p", pr.P2pArgs(seed, mode, 2, 1, 0, nbytes, sb.data_ptr() + offset,
+                                 rb.data_ptr() + offset, sec.data_ptr(), mis.data_ptr()))
+        assert rt.copy_stage_bytes() == (stage_kib << 10 if stage_kib > 0 else 0)
+    assert mis.cpu().tolist() == [0] * P
+    recv = rb.cpu().numpy()[offset:offset + P * nbytes].reshape(P, nbytes)
+    for r in range(P):
+        origin = (r if r % 2 == 0 else r ^ 1) if mode < 2 else (r - 1) % P
+        assert np.array_equal(recv[r], fill_bytes(oracle, seed, origin, nbytes)), r
+
+
+def test_copy_stage_follows_residency(pkg, torch_gpu):
+    """The automatic stage is 32 KiB per rank while the grid keeps its residency headroom
+    with it; 16 KiB for a 1024-rank grid (32 KiB would cap the chip at 5 waves per CU) and
+    for 4 ranks per workgroup (4 x 32 KiB would pass 64 KiB of dynamic LDS)."""
+    from dphpc_project_amd import programs as pr
+    torch = torch_gpu
+    for P, rpb, want in ((256, 1, 32 << 10), (1024, 1, 16 << 10), (256, 4, 16 << 10)):
+        print(f"[gpu] copy stage P={P} rpb={rpb}", flush=True)
+        sb = torch.zeros(P * 64, dtype=torch.uint8, device="cuda")
+        rb = torch.zeros_like(sb)
+        sec = torch.zeros(P, dtype=torch.float64, device="cuda")
+        mis = torch.full((P,), -1, dtype=torch.int64, device="cuda")
+        with pkg.Runtime(P, ranks_per_block=rpb, heap_bytes=64 << 20, timeout_s=20.0) as rt:
+            rt.run("p2p", pr.P2pArgs(0x9, 0, 1, 0, 0, 64, sb.data_ptr(), rb.data_ptr(),
+                                     sec.data_ptr(), mis.data_ptr()))
+            assert rt.copy_stage_bytes() == want, (P, rpb)
+        assert mis.cpu().tolist() == [0] * P
+
+
 def run_coll(pkg, torch, P, kind, dtype, op, count, root=0, seed=0x1234, **cfg):
     from dphpc_project_amd import programs as pr
     es = 8 if dtype == MPI_DOUBLE else 4
