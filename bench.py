@@ -27,7 +27,8 @@ latency (arrival -> result on the host) and jobs/s for the device-resident serve
 versus one plain rank-grid launch per job; every served result is compared with the
 same job run alone on the plain path.  Then the point-to-point sweep (256 ranks,
 ping-pong and ring, 8 B .. 64 MiB), the collective sweep (1024 ranks, 1 .. 2^22 elements)
-with the allocator churn, and NPB IS class C through the batched exchange as a known-answer
+with the allocator churn, the batched point-to-point step (128 pairs of 64 MiB, 65536
+pairs of 4 KiB), and NPB IS class C through the batched exchange as a known-answer
 cross-check at the workload's size.
 """
 import argparse
@@ -628,6 +629,52 @@ def batched_reduction(pkg, torch, args):
     return out
 
 
+def batched_p2p_pairs(pkg, torch, args):
+    """Batched point-to-point step (gpumpi_p2p_*) on two shapes: the 256-rank ping-pong's
+    messages as one step (128 pairs of 64 MiB) and many small messages (65536 pairs of
+    4 KiB).  Each pair i copies source block i to destination block (i * 7919) mod n, so
+    neighbouring pairs write far apart.  Steps replayed from graphs of 10; algorithmic
+    bytes 2 x the pairs' bytes per step.  Sampled pairs are compared byte for byte."""
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for name, npairs, size in (("128_pairs_64MiB", 128, 64 << 20), ("65536_pairs_4KiB", 65536, 4 << 10)):
+        src = torch.randint(0, 256, (npairs * size,), dtype=torch.uint8, device="cuda")
+        dst = torch.zeros_like(src)
+        idx = torch.arange(npairs, dtype=torch.int64, device="cuda")
+        to = (idx * 7919) % npairs
+        pairs = torch.stack([src.data_ptr() + idx * size, dst.data_ptr() + to * size,
+                             torch.full_like(idx, size)], dim=1).contiguous()
+        plan = pkg.P2pPlan(npairs)
+        try:
+            plan.execute(pairs.data_ptr(), npairs, stream)
+            torch.cuda.synchronize()
+            for i in (0, 1, npairs // 2, npairs - 1):
+                j = (i * 7919) % npairs
+                assert torch.equal(dst[j * size:(j + 1) * size], src[i * size:(i + 1) * size]), (name, i)
+            steps = 20
+            run = lambda k: [plan.graph_replay(pairs.data_ptr(), npairs, GRAPH_STEPS, stream)
+                             for _ in range(k // GRAPH_STEPS)]
+            run(GRAPH_STEPS)
+            ms = time_steps(torch, lambda: run(steps), 1) / steps
+            scan_us, copy_us = plan.kernel_times(pairs.data_ptr(), npairs, 5, stream)
+        finally:
+            plan.close()
+        algo = 2 * npairs * size
+        gbs = algo / (ms * 1e-3) / 1e9
+        kgbs = algo / (copy_us * 1e-6) / 1e9
+        out[name] = {"pairs": npairs, "bytes_per_pair": size, "ms_per_step": round(ms, 4),
+                     "GBps": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                     "kernel_us": {"scan": round(scan_us, 2), "copy": round(copy_us, 2)},
+                     "roofline": {"bound": "hbm", "kernel": "p2p_copy_kernel", "achieved": round(kgbs, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(kgbs / HBM_PEAK_GBS, 4)}}
+        log(f"batched p2p {npairs} pairs x {size} B: {ms * 1e3:.1f} us/step {gbs:.0f} GB/s "
+            f"({100 * gbs / HBM_PEAK_GBS:.1f}% of peak); scan {scan_us:.1f} us, copy {copy_us:.1f} us")
+        del src, dst, pairs, idx, to
+        torch.cuda.empty_cache()
+    return out
+
+
 def serve_latency(pkg, torch, args):
     """Poisson arrivals of independent MPI jobs; latency = arrival -> result visible on
     the host, for one plain rank-grid launch per job and for the device-resident server.
@@ -940,6 +987,7 @@ def main():
             f"{ex_ms / 10:.3f} ms per iteration's exchange")
         assert passed == 5 * 10 + 512, "NPB IS class C failed its verification"
     reduction = None if args.no_reduce else batched_reduction(pkg, torch, args)
+    bp2p = None if args.no_reduce else batched_p2p_pairs(pkg, torch, args)
     particles = None if args.no_particles else particle_exchange(pkg, torch, args)
     serve = None if args.no_serve else serve_latency(pkg, torch, args)
     p2p = None if args.no_sweeps else p2p_sweep(pkg, torch, args)
@@ -992,6 +1040,7 @@ def main():
         "zipf": zipf,
         "npb_is": npb,
         "batched_allreduce": reduction,
+        "batched_p2p": bp2p,
         "particles": particles,
         "serving": serve,
         "p2p": p2p,

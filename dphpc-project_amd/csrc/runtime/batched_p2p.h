@@ -37,24 +37,47 @@ constexpr int kP2pU = (int)(kP2pChunk / 16 / kWave);     // 16-byte vectors per 
 constexpr int kP2pScanThreads = 1024;
 constexpr int kP2pThreads = 256;
 
+// Branch-free, so that callers can issue the loads of many pairs before using any.
 __device__ __forceinline__ uint64_t p2p_chunks(const char* dst, uint64_t bytes) {
-    if (bytes == 0) return 0;
     const uint64_t a = (uint64_t)(uintptr_t)dst;
-    return ((a + bytes - 1) >> kP2pChunkShift) - (a >> kP2pChunkShift) + 1;
+    const uint64_t c = ((a + bytes - 1) >> kP2pChunkShift) - (a >> kP2pChunkShift) + 1;
+    return bytes ? c : 0;
 }
 
 // chunk_off[i] = chunks of pairs 0 .. i-1, for i = 0 .. n (chunk_off[n] = all chunks).
-// Passes of 1024 pairs, one per thread: wave scans, wave totals through LDS, a carry.
+// Passes of 7168 pairs through LDS (56 KiB): the pairs are read coalesced (pair
+// k * 1024 + t by thread t) and their chunk counts parked in LDS; thread t then scans
+// counts 7t .. 7t+6, the thread sums are scanned across the workgroup (wave scans, wave
+// totals through LDS), and the exclusive offsets go back to LDS and out to chunk_off
+// coalesced again.  With one pair
+// per thread and pass, 65536 pairs took 64 dependent passes (129 us); with 16 consecutive
+// pairs per thread the loads were uncoalesced (99 us).
+constexpr int kP2pScanPer = 7;
+constexpr int kP2pScanPass = kP2pScanThreads * kP2pScanPer;
 static __global__ __launch_bounds__(kP2pScanThreads) void p2p_scan_kernel(const P2pPair* pairs, int64_t n,
                                                                    int64_t* chunk_off) {
     constexpr int kWaves = kP2pScanThreads / kWave;
+    __shared__ int64_t s_c[kP2pScanPass];
     __shared__ int64_t s_w[kWaves];
     const int t = threadIdx.x, wv = t >> 6, l = t & 63;
     int64_t carry = 0;
-    for (int64_t i0 = 0; i0 < n; i0 += kP2pScanThreads) {
-        const int64_t i = i0 + t;
-        const int64_t c = i < n ? (int64_t)p2p_chunks(pairs[i].dst, pairs[i].bytes) : 0;
-        int64_t incl = c;
+    for (int64_t i0 = 0; i0 < n; i0 += kP2pScanPass) {
+        // a clamped index (n >= 1 here) and a masked size: no branch to sink the loads into
+#pragma unroll
+        for (int k = 0; k < kP2pScanPer; k++) {
+            const int64_t j = i0 + (int64_t)k * kP2pScanThreads + t;
+            const P2pPair p = pairs[j < n ? j : n - 1];
+            s_c[k * kP2pScanThreads + t] = (int64_t)p2p_chunks(p.dst, j < n ? p.bytes : 0);
+        }
+        __syncthreads();
+        int64_t c[kP2pScanPer];
+        int64_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kP2pScanPer; k++) {
+            c[k] = s_c[t * kP2pScanPer + k];
+            sum += c[k];
+        }
+        int64_t incl = sum;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
             const int64_t u = __shfl_up(incl, o);
@@ -67,9 +90,20 @@ static __global__ __launch_bounds__(kP2pScanThreads) void p2p_scan_kernel(const 
             before += u < wv ? s_w[u] : 0;
             all += s_w[u];
         }
-        if (i < n) chunk_off[i] = carry + before + incl - c;
+        int64_t run = carry + before + incl - sum;
+#pragma unroll
+        for (int k = 0; k < kP2pScanPer; k++) {
+            s_c[t * kP2pScanPer + k] = run;
+            run += c[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kP2pScanPer; k++) {
+            const int64_t j = i0 + (int64_t)k * kP2pScanThreads + t;
+            if (j < n) chunk_off[j] = s_c[k * kP2pScanThreads + t];
+        }
         carry += all;
-        __syncthreads();   // s_w is rewritten by the next pass
+        __syncthreads();   // s_c and s_w are rewritten by the next pass
     }
     if (t == 0) chunk_off[n] = carry;
 }

@@ -62,15 +62,17 @@ def test_edge_sizes_misaligned(pkg, torch_gpu, seed):
     plan.close()
 
 
-def test_many_small_pairs(pkg, torch_gpu):
-    """5000 pairs of 0-300 bytes: several scan passes of 1024 pairs, and copy waves whose
-    chunk ranges start in the middle of the list."""
+@pytest.mark.parametrize("npairs", [5000, 40001])
+def test_many_small_pairs(npairs, pkg, torch_gpu):
+    """Pairs of 0-300 bytes: 5000 fit one scan pass (7168 pairs), 40001 take six with a
+    part pass and a part thread at the end; copy waves' chunk ranges start in the middle
+    of the list."""
     torch = torch_gpu
     rng = np.random.default_rng(7)
-    sizes = rng.integers(0, 301, size=5000)
-    print("[gpu] p2p 5000 small pairs", flush=True)
+    sizes = rng.integers(0, 301, size=npairs)
+    print(f"[gpu] p2p {npairs} small pairs", flush=True)
     src, dst, exp, pairs = build(torch, rng, sizes)
-    plan = pkg.P2pPlan(8192)
+    plan = pkg.P2pPlan(npairs + 100)
     plan.execute(pairs.data_ptr(), len(sizes))
     check(torch, dst, exp)
 
