@@ -116,7 +116,7 @@ __device__ __forceinline__ void lane_copy_to_private(char* d, const char* s, uin
 extern __shared__ __attribute__((aligned(16))) char gpumpi_copy_stage[];
 __shared__ uint32_t s_stage_bytes;
 
-constexpr uint64_t kStagedMinBytes = 64u << 10;   // smaller copies stay in registers
+constexpr uint64_t kStagedMinBytes = 16u << 10;   // smaller copies stay in registers
 
 // Copies the whole stage-sized batches of d[0 .. n) from s (both 16-byte aligned); returns
 // the bytes done.  stage_bytes is a multiple of 1 KiB (one 16-byte load per lane).

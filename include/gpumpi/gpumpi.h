@@ -62,7 +62,7 @@ typedef struct gpumpi_config {
                                    least 512 KiB in all (the measured crossover on 64 and
                                    1024 ranks), < 0 never */
     int32_t copy_stage_kib;     /* LDS staging area per rank (wavefront) for its copies of
-                                   64 KiB and more: 0 = the largest of 32 and 16 KiB with
+                                   16 KiB and more: 0 = the largest of 32 and 16 KiB with
                                    which the grid keeps its residency headroom (none if
                                    neither does), > 0 = this many KiB if it fits, < 0 never.
                                    The launch gives it as dynamic LDS. */

@@ -98,13 +98,15 @@ def test_p2p_256_ranks(mode, nbytes, pkg, oracle, torch_gpu):
 
 
 @pytest.mark.parametrize("stage_kib", [-1, 16, 32])
-@pytest.mark.parametrize("offset,nbytes", [(0, 1 << 20), (5, 3 * 32768 + 16 * 7), (0, 65536 + 16)])
+@pytest.mark.parametrize("offset,nbytes", [(0, 1 << 20), (5, 3 * 32768 + 16 * 7), (0, 65536 + 16),
+                                           (3, 16384 + 80)])
 @pytest.mark.parametrize("mode", [0, 3])
 def test_p2p_lds_staged_copies(mode, offset, nbytes, stage_kib, pkg, oracle, torch_gpu):
     """Large messages through the LDS-staged copy (copy_stage_kib 16 / 32) and through
     registers only (-1), byte-exact against the oracle.  Buffers at offset 5 give every
     message an 11-byte head before the 16-byte aligned part and a byte tail after it;
-    3 x 32 KiB + 112 leaves a part batch after the whole stage batches."""
+    3 x 32 KiB + 112 leaves a part batch after the whole stage batches; 16 KiB + 80 at
+    offset 3 is one whole 16 KiB batch, or none of 32 KiB, around a 13-byte head."""
     from dphpc_project_amd import programs as pr
     torch = torch_gpu
     P, seed = 16, 0x5ba
