@@ -111,7 +111,7 @@ __device__ __forceinline__ void lane_copy_to_private(char* d, const char* s, uin
 // loads (global_load_lds_dwordx4, 16 bytes per lane, no VGPR holds the data in flight),
 // waits for them, and moves the area to the destination through ds_read and global
 // stores.  Measured with 128 waves copying 64 MiB each at once: 10.8 GB/s per wave through
-// registers (8 KiB in flight), 14.4 / 16.7 GB/s through a 16 / 32 KiB stage.
+// registers (8 KiB in flight), 14.4 / 16.7 / 19.1 GB/s through a 16 / 32 / 64 KiB stage.
 #define GPUMPI_LDS __attribute__((address_space(3)))
 extern __shared__ __attribute__((aligned(16))) char gpumpi_copy_stage[];
 __shared__ uint32_t s_stage_bytes;

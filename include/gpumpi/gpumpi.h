@@ -62,10 +62,11 @@ typedef struct gpumpi_config {
                                    least 512 KiB in all (the measured crossover on 64 and
                                    1024 ranks), < 0 never */
     int32_t copy_stage_kib;     /* LDS staging area per rank (wavefront) for its copies of
-                                   16 KiB and more: 0 = the largest of 32 and 16 KiB with
-                                   which the grid keeps its residency headroom (none if
-                                   neither does), > 0 = this many KiB if it fits, < 0 never.
-                                   The launch gives it as dynamic LDS. */
+                                   16 KiB and more: 0 = the largest of 64, 32 and 16 KiB
+                                   with which the grid keeps its residency headroom and the
+                                   workgroup fits the device's LDS limit (none if no size
+                                   does), 1..64 = this many KiB if it fits, < 0 never.  The
+                                   launch gives it as dynamic LDS. */
     int32_t reserved1;          /* must be 0 */
     uint64_t heap_bytes;        /* -p: device heap arena; 0 = 1 GiB */
     uint64_t stack_bytes;       /* -s: per-lane stack limit; 0 = runtime default */

@@ -97,12 +97,12 @@ def test_p2p_256_ranks(mode, nbytes, pkg, oracle, torch_gpu):
         assert np.array_equal(got, fill_bytes(oracle, seed, origin, nbytes)), r
 
 
-@pytest.mark.parametrize("stage_kib", [-1, 16, 32])
+@pytest.mark.parametrize("stage_kib", [-1, 16, 32, 64])
 @pytest.mark.parametrize("offset,nbytes", [(0, 1 << 20), (5, 3 * 32768 + 16 * 7), (0, 65536 + 16),
                                            (3, 16384 + 80)])
 @pytest.mark.parametrize("mode", [0, 3])
 def test_p2p_lds_staged_copies(mode, offset, nbytes, stage_kib, pkg, oracle, torch_gpu):
-    """Large messages through the LDS-staged copy (copy_stage_kib 16 / 32) and through
+    """Large messages through the LDS-staged copy (copy_stage_kib 16 / 32 / 64) and through
     registers only (-1), byte-exact against the oracle.  Buffers at offset 5 give every
     message an 11-byte head before the 16-byte aligned part and a byte tail after it;
     3 x 32 KiB + 112 leaves a part batch after the whole stage batches; 16 KiB + 80 at
@@ -118,7 +118,9 @@ def test_p2p_lds_staged_copies(mode, offset, nbytes, stage_kib, pkg, oracle, tor
     with pkg.Runtime(P, heap_bytes=64 << 20, timeout_s=20.0, copy_stage_kib=stage_kib) as rt:
         rt.run("p2p", pr.P2pArgs(seed, mode, 2, 1, 0, nbytes, sb.data_ptr() + offset,
                                  rb.data_ptr() + offset, sec.data_ptr(), mis.data_ptr()))
-        assert rt.copy_stage_bytes() == (stage_kib << 10 if stage_kib > 0 else 0)
+        spb = torch.cuda.get_device_properties(0).shared_memory_per_block
+        fits = stage_kib > 0 and (stage_kib << 10) + 1024 <= spb
+        assert rt.copy_stage_bytes() == (stage_kib << 10 if fits else 0)
     assert mis.cpu().tolist() == [0] * P
     recv = rb.cpu().numpy()[offset:offset + P * nbytes].reshape(P, nbytes)
     for r in range(P):
@@ -127,12 +129,22 @@ def test_p2p_lds_staged_copies(mode, offset, nbytes, stage_kib, pkg, oracle, tor
 
 
 def test_copy_stage_follows_residency(pkg, torch_gpu):
-    """The automatic stage is 32 KiB per rank while the grid keeps its residency headroom
-    with it; 16 KiB for a 1024-rank grid (32 KiB would cap the chip at 5 waves per CU) and
-    for 4 ranks per workgroup (4 x 32 KiB would pass 64 KiB of dynamic LDS)."""
+    """The automatic stage is the largest of 64 / 32 / 16 KiB per rank with which the grid
+    keeps its residency headroom (half of the co-resident wave slots) and the workgroup's
+    LDS fits the device's per-workgroup limit.  On 256 CUs with 160 KiB of LDS each and a
+    program of two waves per SIMD: 256 ranks take 64 KiB (2 workgroups per CU), 512 ranks
+    32 KiB (5 per CU), 1024 ranks 16 KiB (8 per CU); 4 ranks per workgroup take 32 KiB
+    when a workgroup may hold 128 KiB.  Where the device allows less per workgroup, the
+    next size down."""
     from dphpc_project_amd import programs as pr
     torch = torch_gpu
-    for P, rpb, want in ((256, 1, 32 << 10), (1024, 1, 16 << 10), (256, 4, 16 << 10)):
+    spb = torch.cuda.get_device_properties(0).shared_memory_per_block
+    slack = 1024   # the program's own static LDS (a few pointers)
+    cases = ((256, 1, 64 << 10 if spb >= (64 << 10) + slack else 32 << 10),
+             (512, 1, 32 << 10), (1024, 1, 16 << 10),
+             (256, 4, 32 << 10 if spb >= (128 << 10) + slack else 16 << 10))
+    print(f"[gpu] LDS per workgroup {spb}", flush=True)
+    for P, rpb, want in cases:
         print(f"[gpu] copy stage P={P} rpb={rpb}", flush=True)
         sb = torch.zeros(P * 64, dtype=torch.uint8, device="cuda")
         rb = torch.zeros_like(sb)
@@ -141,7 +153,7 @@ def test_copy_stage_follows_residency(pkg, torch_gpu):
         with pkg.Runtime(P, ranks_per_block=rpb, heap_bytes=64 << 20, timeout_s=20.0) as rt:
             rt.run("p2p", pr.P2pArgs(0x9, 0, 1, 0, 0, 64, sb.data_ptr(), rb.data_ptr(),
                                      sec.data_ptr(), mis.data_ptr()))
-            assert rt.copy_stage_bytes() == want, (P, rpb)
+            assert rt.copy_stage_bytes() == want, (P, rpb, rt.copy_stage_bytes())
         assert mis.cpu().tolist() == [0] * P
 
 
