@@ -2071,7 +2071,9 @@ int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_
     if (gpumpi_device_info(&cus, nullptr, nullptr) != GPUMPI_OK) return GPUMPI_E_HIP;
     const uint64_t bytes = es > 0 ? (uint64_t)count * (uint64_t)es : 0;
     const bool wide = bytes / (1024ull * kReduceWideU) >= 4ull * (uint64_t)cus;
-    const void* k = es > 0 ? batched_reduce_kernel_lookup(datatype, op, wide) : nullptr;
+    // integer pairs below the wide shape fold with the ranks split over a workgroup's waves
+    const void* ks = es > 0 && !wide && P >= kReduceSplitWaves ? batched_reduce_split_lookup(datatype, op) : nullptr;
+    const void* k = ks ? ks : es > 0 ? batched_reduce_kernel_lookup(datatype, op, wide) : nullptr;
     if (!k) {
         g_host_error = "batched reduction: the pair (datatype " + std::to_string(datatype) +
                        ", operator " + std::to_string(op) + ") is not on the batched path "
@@ -2088,7 +2090,8 @@ int gpumpi_reduce_plan_create(int32_t P, int64_t count, int32_t datatype, int32_
     pl->dev.nt = written >= (64ull << 20) ? 1 : 0;  // outputs larger than the L2s stream past them
     pl->dev.sendbufs = (const char*)sendbufs;
     pl->dev.recvbufs = (const char*)recvbufs;
-    uint64_t g = batched_reduce_grid(bytes, wide ? kReduceWideU : kReduceNarrowU);
+    uint64_t g = ks ? batched_reduce_split_grid(bytes)
+                    : batched_reduce_grid(bytes, wide ? kReduceWideU : kReduceNarrowU);
     if (g > (1u << 20)) g = 1u << 20;  // the waves stride over the rest
     pl->grid = (int)g;
     *out = pl;

@@ -25,5 +25,6 @@ const void* a2av_copy_kernel_for(int U, bool nt, bool defer, bool fuse);
 // kernels_reduce.hip: the batched reduction kernel of a (datatype, built-in operator) pair
 // in the narrow or wide shape, or nullptr.
 const void* batched_reduce_kernel_lookup(int dt, int op, bool wide);
+const void* batched_reduce_split_lookup(int dt, int op);
 
 }  // namespace gpumpi
