@@ -611,7 +611,11 @@ def batched_reduction(pkg, torch, args):
             out[key] = {"count_per_rank": n, "ms_per_step": round(ms, 4), "GBps": round(gbs, 1),
                         "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                         "kernel_us": round(kus, 2),
-                        "roofline": {"bound": "hbm", "kernel": "batched_reduce_kernel",
+                        # integer pairs below 8 MiB per rank run the split fold
+                        "roofline": {"bound": "hbm",
+                                     "kernel": ("batched_reduce_split_kernel"
+                                                if dname != "MPI_DOUBLE" and per_rank < (8 << 20)
+                                                else "batched_reduce_kernel"),
                                      "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                      "frac": round(kgbs / HBM_PEAK_GBS, 4)},
                         "plain_ms_per_call": round(plain_ms, 4),
