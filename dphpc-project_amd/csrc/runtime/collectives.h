@@ -345,6 +345,53 @@ __device__ int coll_reduce(const void* sendbuf, void* recvbuf, int64_t count, in
 // ------------------------------------------------------------------ Alltoall(v)
 constexpr int64_t kAlltoallLaneBytes = 16;   // blocks up to this size: pulled lane by lane
 
+// A uniform Alltoall's pull for blocks of whole 16-byte vectors, at most VPL per lane
+// (n <= 1 KiB * VPL), from sources (me + k) mod P, k = 1 .. P.  The send buffers of 64
+// sources at a time are read into the lanes (one gather) and taken out with readlane;
+// each pass then loads the blocks of G = 8 / VPL sources before storing any, so it waits
+// one HBM latency for G blocks where a block-by-block pull waits one per block (two,
+// counting the read of the source's slot).  A sender's block for rank me starts at
+// me * n: a uniform Alltoall's send and receive blocks have the same bytes.  The
+// destination rb + s * n is 16-byte aligned; sources may be unaligned.  Out of line, so
+// its 8 vectors of loads stay out of the rank programs' register count.
+template <int VPL>
+__device__ __noinline__ void alltoall_pull_uniform(char* rb, const CollSlot* slots, int me, int P,
+                                                   uint64_t n) {
+    constexpr int G = 8 / VPL;   // divides 64
+    const int l = lane();
+    const uint64_t nv = n >> 4;
+    const uint64_t soff = (uint64_t)me * n;
+    for (int k0 = 1; k0 <= P; k0 += kWave) {
+        const int nk = P - k0 + 1 < kWave ? P - k0 + 1 : kWave;
+        const uint64_t sb = l < nk ? (uint64_t)(uintptr_t)slots[(me + k0 + l) % P].sbuf : 0;
+        for (int j0 = 0; j0 < nk; j0 += G) {
+            u32x4 v[G][VPL];
+#pragma unroll
+            for (int b = 0; b < G; b++) {
+                if (j0 + b >= nk) break;   // wave-uniform
+                const uint64_t sp = readlane64(sb, j0 + b) + soff;
+#pragma unroll
+                for (int u = 0; u < VPL; u++) {
+                    const uint64_t i = (uint64_t)u * kWave + (uint64_t)l;
+                    v[b][u] = gload16(sp + 16 * (i < nv ? i : 0));   // clamped: every lane loads
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < G; b++) {
+                if (j0 + b >= nk) break;
+                int s = me + k0 + j0 + b;   // < 2P
+                s -= s >= P ? P : 0;
+                const uint64_t dp = (uint64_t)(uintptr_t)rb + (uint64_t)s * n;
+#pragma unroll
+                for (int u = 0; u < VPL; u++) {
+                    const uint64_t i = (uint64_t)u * kWave + (uint64_t)l;
+                    if (i < nv) gstore16<false>(dp + 16 * i, v[b][u]);
+                }
+            }
+        }
+    }
+}
+
 // counts/displs are in elements of the respective datatype (gpu_mpi/mpi.cu:363-408).
 __device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int* sdispls,
                               int sdt, void* recvbuf, const int* rcounts, const int* rdispls,
@@ -421,7 +468,20 @@ __device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int
             }
         }
     }
-    for (int k = 1; k <= P && !small; k++) {
+    // Uniform blocks of whole 16-byte vectors up to 4 KiB into an aligned global buffer:
+    // several sources per pass (alltoall_pull_uniform)
+    bool batched = false;
+    if (!small && uniform && !needs_staging(rb) && ((uintptr_t)rb & 15) == 0) {
+        const uint64_t n = (uint64_t)uniform_rcount * re;
+        const uint64_t vpl = (n / 16 + kWave - 1) / kWave;
+        if (n != 0 && n % 16 == 0 && vpl <= 4) {
+            batched = true;
+            if (vpl == 1) alltoall_pull_uniform<1>((char*)rb, c->slots, me, P, n);
+            else if (vpl == 2) alltoall_pull_uniform<2>((char*)rb, c->slots, me, P, n);
+            else alltoall_pull_uniform<4>((char*)rb, c->slots, me, P, n);
+        }
+    }
+    for (int k = 1; k <= P && !small && !batched; k++) {
         const int s = (me + k) % P;
         const CollSlot* src = &c->slots[s];
         uint64_t n, soff, doff;

@@ -116,6 +116,34 @@ def test_alltoall_small_blocks_1024_ranks(dtype, per, pkg, oracle, torch_gpu):
             assert bad.size == 0, f"destinations {bad[:8]} differ"
 
 
+@pytest.mark.parametrize("dtype,per", [(MPI_INT, 16), (MPI_INT, 256), (MPI_INT, 512), (MPI_INT, 768),
+                                       (MPI_INT, 1024), (MPI_INT, 1028), (MPI_DOUBLE, 8),
+                                       (MPI_DOUBLE, 3), (MPI_DOUBLE, 128), (MPI_DOUBLE, 512)])
+def test_alltoall_blocks_1024_ranks_transpose(dtype, per, pkg, torch_gpu):
+    """MPI_Alltoall of 64 B .. 4 KiB blocks: whole 16-byte vectors up to 4 KiB are pulled
+    several sources per pass (1, 2 or 4 vectors per lane: 8, 4 or 2 sources); 4112-byte and
+    24-byte blocks take the one-source-per-pass copy.  Every destination's receive buffer
+    equals the block transpose of the inputs (which the program generated on the device),
+    compared on the device; twice per runtime."""
+    torch = torch_gpu
+    print(f"[gpu] alltoall 1024 ranks dtype={dtype} per={per} (transpose)", flush=True)
+    count = per * P
+    tdt = torch.float64 if dtype == MPI_DOUBLE else torch.int32
+    inb = torch.zeros((P, count), dtype=tdt, device="cuda")
+    outb = torch.zeros((P, count), dtype=tdt, device="cuda")
+    with _runtime(pkg) as rt:
+        for _ in range(2):
+            outb.zero_()
+            _run(pkg, rt, torch, "alltoall", dtype, OP_SUM, count, 0, inb, outb)
+            exp = inb.view(P, P, per).transpose(0, 1).contiguous().view(P, count)
+            same = (outb == exp).all(dim=1)
+            bad = torch.nonzero(~same).flatten()[:8].tolist()
+            assert not bad, f"destinations {bad} differ"
+            del exp, same
+    del inb, outb
+    torch.cuda.empty_cache()
+
+
 def _sample_indices(count, rng):
     """Both ends of every rank's reduction slice (multiples of 256 elements, the runtime's
     slicing) plus random elements."""
