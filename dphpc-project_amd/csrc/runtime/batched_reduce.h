@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kReduceThreads) batched_reduce_kernel(ReduceDe
         for (int u = 0; u < U; u++)
             if (v0 + (uint64_t)kWave * u < nvec) valid |= 1u << u;
         u32x4 acc[U];
-        fold_columns<T, OP, U, D>(src, p.P, 16 * v0, valid, acc);
+        fold_columns_unmasked<T, OP, U, D>(src, p.P, 16 * v0, valid, acc);
         if (p.nt) fold_store_cols<true, U>(dst, p.P, one, 16 * v0, valid, acc);
         else fold_store_cols<false, U>(dst, p.P, one, 16 * v0, valid, acc);
     }
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kReduceThreads) batched_reduce_split_kernel(Re
         for (int u = 0; u < U; u++)
             if (v0 + (uint64_t)kWave * u < nvec) valid |= 1u << u;
         u32x4 acc[U];
-        fold_columns<T, OP, U, D>(src, r1 - r0, 16 * v0, valid, acc);
+        fold_columns_unmasked<T, OP, U, D>(src, r1 - r0, 16 * v0, valid, acc);
 #pragma unroll
         for (int u = 0; u < U; u++) part[q][u][l] = acc[u];
         __syncthreads();

@@ -20,9 +20,17 @@ namespace gpumpi {
 struct PtrTable {
     const char* base;
     int64_t stride;
+    // Entries past P read entry P - 1 and return 0.  The load is unconditional and in the
+    // global address space: the compiler counts a load in flight only when it is neither a
+    // flat load (whose completion is not ordered with the other counters) nor issued under
+    // a lane mask that may skip it; otherwise each readlane of the result waits for every
+    // load and store in flight (s_waitcnt vmcnt(0)) -- which drained the fold's ring of
+    // loads at every rank.
     __device__ __forceinline__ uint64_t lane_chunk(int r0, int P) const {
         const int r = r0 + lane();
-        return r < P ? *(const uint64_t*)(base + (int64_t)r * stride) : 0;
+        const int rc = r < P ? r : P - 1;
+        const uint64_t v = *(const GPUMPI_GLOBAL uint64_t*)(base + (int64_t)rc * stride);
+        return r < P ? v : 0;
     }
 };
 
@@ -87,6 +95,64 @@ __device__ __forceinline__ void fold_columns(const PtrTable& src, int P, uint64_
         }
         pc = pn;
         pn = r0 + 2 * kWave < P ? src.lane_chunk(r0 + 2 * kWave, P) : 0;
+    }
+}
+
+// fold_columns for the batched kernels, where registers are plentiful: the same fold with
+// no load under a branch.  A column a lane does not fold loads lane 0's first column of the
+// same rank instead, ranks past P - 1 load rank P - 1, and the fold takes a value or keeps
+// acc by selection.  The compiler can then count the loads in flight and wait for the
+// oldest only (s_waitcnt vmcnt(N)), so D ranks' loads stay in flight; in fold_columns the
+// masked loads make it wait for all of them (vmcnt(0)) before each rank's readlane.
+// Wave-uniform P >= 1; lane 0's column 0 must be valid (the callers' tiles start at a valid
+// column).  The rank grid keeps fold_columns: this one's extra registers, through the
+// interprocedural allocation, took rank programs past 256 per lane.
+template <class T, int OP, int U, int D>
+__device__ __forceinline__ void fold_columns_unmasked(const PtrTable& src, int P, uint64_t off,
+                                                      uint32_t valid, u32x4 (&acc)[U]) {
+    static_assert(D >= 1 && D <= kWave && kWave % D == 0, "D must divide the wave size");
+    const uint64_t off0 = bcast(off);   // lane 0's first column
+    uint64_t co[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) co[u] = (valid >> u & 1u) ? off + 1024u * u : off0;
+    u32x4 ring[D][U];
+    uint64_t pc = src.lane_chunk(0, P);
+    uint64_t pn = src.lane_chunk(kWave, P);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const uint64_t b = readlane64(pc, k < P ? k : P - 1);
+#pragma unroll
+        for (int u = 0; u < U; u++) ring[k][u] = gload16(b + co[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = u32x4{0u, 0u, 0u, 0u};
+    for (int r0 = 0; r0 < P; r0 += kWave) {
+        for (int j0 = 0; j0 < kWave; j0 += D) {
+            if (r0 + j0 >= P) break;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const int r = r0 + j0 + k;
+                u32x4 x[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) x[u] = ring[k][u];
+                // refill with rank r + D (rank P - 1 past the end): its address is in this
+                // chunk's pointers or the next chunk's
+                const int rn = r + D < P ? r + D : P - 1;
+                const int jn = rn - r0;
+                const uint64_t bc = readlane64(pc, jn < kWave ? jn : 0);
+                const uint64_t bn = readlane64(pn, jn < kWave ? 0 : jn - kWave);
+                const uint64_t b = jn < kWave ? bc : bn;
+#pragma unroll
+                for (int u = 0; u < U; u++) ring[k][u] = gload16(b + co[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const u32x4 f = r == 0 ? x[u] : fold_vec_op<T, OP>(x[u], acc[u]);
+                    acc[u] = r < P ? f : acc[u];
+                }
+            }
+        }
+        pc = pn;
+        pn = src.lane_chunk(r0 + 2 * kWave, P);
     }
 }
 
