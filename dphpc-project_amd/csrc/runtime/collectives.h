@@ -265,6 +265,23 @@ struct ReduceFunctor {
         const int64_t b0 = lo * (int64_t)sizeof(T);
         const int64_t nv = sizeof(T) >= 4 ? (hi - lo) * (int64_t)sizeof(T) / 16 : 0;
         constexpr int64_t kTile = (int64_t)kWave * kFoldU;
+        if constexpr (sizeof(T) >= 4) {
+            if (nv > 0 && nv <= (int64_t)kWave * kFoldSmallU) {
+                // one small tile: the unmasked fold
+                const int64_t v = lane();
+                uint32_t valid = 0;
+#pragma unroll
+                for (int u = 0; u < kFoldSmallU; u++)
+                    if (v + (int64_t)kWave * u < nv) valid |= 1u << u;
+                const uint64_t off = (uint64_t)(b0 + 16 * v);
+                u32x4 acc[kFoldSmallU];
+                fold_columns_unmasked<T, OP, kFoldSmallU, kFoldSmallD>(src, P, off, valid, acc);
+                if (nt) fold_store_cols<true, kFoldSmallU>(dst, P, one, off, valid, acc);
+                else fold_store_cols<false, kFoldSmallU>(dst, P, one, off, valid, acc);
+                fold_scalar_tail<T, OP>(src, P, dst, P, one, lo + nv * 16 / (int64_t)sizeof(T), hi);
+                return;
+            }
+        }
         for (int64_t v0 = 0; v0 < nv; v0 += kTile) {
             const int64_t v = v0 + lane();
             uint32_t valid = 0;

@@ -163,6 +163,13 @@ __device__ __forceinline__ void fold_columns_unmasked(const PtrTable& src, int P
 // registers per lane.
 constexpr int kFoldU = 4;
 constexpr int kFoldD = 4;
+// Slices of at most one tile of 2 columns per lane (Reduce at most 2 KiB per rank and
+// slice) fold with fold_columns_unmasked, 2 columns x 4 ranks: latency-bound there, they
+// gain from the ring of loads staying in flight.  Same-process A/B on 1024 ranks at 2^16
+// elements: Reduce 427 -> 211 us (MPI_INT), 532 -> 248 us (MPI_DOUBLE), Allreduce 602 -> 320
+// / 708 -> 358 us; at 2^20 and 2^22 the 4 x 4 fold above was 4-15 % faster.
+constexpr int kFoldSmallU = 2;
+constexpr int kFoldSmallD = 4;
 
 // Stores a lane's folded column to every destination of the table (ndst of them), or to
 // one destination address when `one` is non-zero.
