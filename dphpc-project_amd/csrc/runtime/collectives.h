@@ -58,11 +58,12 @@ constexpr int kBcastU = 16;                        // 16-byte columns per lane i
 // Calls f(address of the receiver's buffer) for every receiver (every rank but the root),
 // reading the buffer addresses 64 at a time and loading the next 64 while f runs.
 template <class F>
-__device__ __forceinline__ void for_each_receiver(const PtrTable& dst, int P, int root, F&& f) {
+__device__ __forceinline__ void for_each_receiver(const PtrTable& dst, int P, int root_in, F&& f) {
+    const int root = bcast(root_in);   // wave-uniform: the root test is a scalar branch
     uint64_t pn = dst.lane_chunk(0, P);
     for (int d0 = 0; d0 < P; d0 += kWave) {
         const uint64_t pd = pn;
-        if (d0 + kWave < P) pn = dst.lane_chunk(d0 + kWave, P);
+        pn = dst.lane_chunk(d0 + kWave < P ? d0 + kWave : d0, P);   // unconditional prefetch
         const int n = P - d0 < kWave ? P - d0 : kWave;
         for (int j = 0; j < n; j++)
             if (d0 + j != root) f(readlane64(pd, j));

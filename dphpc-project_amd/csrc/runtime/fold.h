@@ -20,17 +20,17 @@ namespace gpumpi {
 struct PtrTable {
     const char* base;
     int64_t stride;
-    // Entries past P read entry P - 1 and return 0.  The load is unconditional and in the
-    // global address space: the compiler counts a load in flight only when it is neither a
+    // Lanes past P read entry P - 1 (callers read lanes below P - r0 only).  The load is
+    // unconditional, in the global address space, and its value is not touched before
+    // the caller's readlane: the compiler counts a load in flight only when it is neither a
     // flat load (whose completion is not ordered with the other counters) nor issued under
-    // a lane mask that may skip it; otherwise each readlane of the result waits for every
-    // load and store in flight (s_waitcnt vmcnt(0)) -- which drained the fold's ring of
-    // loads at every rank.
+    // a lane mask that may skip it, and it waits where the value is first used.  Otherwise
+    // each readlane of the result waits for every load and store in flight
+    // (s_waitcnt vmcnt(0)) -- which drained the fold's ring of loads at every rank, and a
+    // prefetch of the next 64 entries waited as soon as it was issued.
     __device__ __forceinline__ uint64_t lane_chunk(int r0, int P) const {
         const int r = r0 + lane();
-        const int rc = r < P ? r : P - 1;
-        const uint64_t v = *(const GPUMPI_GLOBAL uint64_t*)(base + (int64_t)rc * stride);
-        return r < P ? v : 0;
+        return *(const GPUMPI_GLOBAL uint64_t*)(base + (int64_t)(r < P ? r : P - 1) * stride);
     }
 };
 
