@@ -363,22 +363,23 @@ __device__ int coll_reduce(const void* sendbuf, void* recvbuf, int64_t count, in
 // ------------------------------------------------------------------ Alltoall(v)
 constexpr int64_t kAlltoallLaneBytes = 16;   // blocks up to this size: pulled lane by lane
 
-// A uniform Alltoall's pull for blocks of whole 16-byte vectors, at most VPL per lane
-// (n <= 1 KiB * VPL), from sources (me + k) mod P, k = 1 .. P.  The send buffers of 64
+// A uniform Alltoall's (or Allgather's) pull for blocks of whole 16-byte vectors, at most
+// VPL per lane (n <= 1 KiB * VPL), from sources (me + k) mod P, k = 1 .. P: source s's
+// bytes [soff, soff + n) go to rb + s * n.  The send buffers of 64
 // sources at a time are read into the lanes (one gather) and taken out with readlane;
 // each pass then loads the blocks of G = 8 / VPL sources before storing any, so it waits
 // one HBM latency for G blocks where a block-by-block pull waits one per block (two,
-// counting the read of the source's slot).  A sender's block for rank me starts at
-// me * n: a uniform Alltoall's send and receive blocks have the same bytes.  The
-// destination rb + s * n is 16-byte aligned; sources may be unaligned.  Out of line, so
-// its 8 vectors of loads stay out of the rank programs' register count.
+// counting the read of the source's slot).  Alltoall: a sender's block for rank me starts
+// at soff = me * n (a uniform Alltoall's send and receive blocks have the same bytes);
+// Allgather: soff = 0.  The destination rb + s * n is 16-byte aligned; sources may be
+// unaligned.  Out of line, so its 8 vectors of loads stay out of the rank programs'
+// register count.
 template <int VPL>
-__device__ __noinline__ void alltoall_pull_uniform(char* rb, const CollSlot* slots, int me, int P,
-                                                   uint64_t n) {
+__device__ __noinline__ void coll_pull_uniform(char* rb, const CollSlot* slots, int me, int P,
+                                               uint64_t n, uint64_t soff) {
     constexpr int G = 8 / VPL;   // divides 64
     const int l = lane();
     const uint64_t nv = n >> 4;
-    const uint64_t soff = (uint64_t)me * n;
     for (int k0 = 1; k0 <= P; k0 += kWave) {
         const int nk = P - k0 + 1 < kWave ? P - k0 + 1 : kWave;
         const uint64_t sb = l < nk ? (uint64_t)(uintptr_t)slots[(me + k0 + l) % P].sbuf : 0;
@@ -408,6 +409,15 @@ __device__ __noinline__ void alltoall_pull_uniform(char* rb, const CollSlot* slo
             }
         }
     }
+}
+
+// coll_pull_uniform for n <= 4 KiB of whole 16-byte vectors.
+__device__ __forceinline__ void coll_pull_blocks(char* rb, const CollSlot* slots, int me, int P,
+                                                 uint64_t n, uint64_t soff) {
+    const uint64_t vpl = (n / 16 + kWave - 1) / kWave;
+    if (vpl == 1) coll_pull_uniform<1>(rb, slots, me, P, n, soff);
+    else if (vpl == 2) coll_pull_uniform<2>(rb, slots, me, P, n, soff);
+    else coll_pull_uniform<4>(rb, slots, me, P, n, soff);
 }
 
 // counts/displs are in elements of the respective datatype (gpu_mpi/mpi.cu:363-408).
@@ -487,16 +497,14 @@ __device__ int coll_alltoallv(const void* sendbuf, const int* scounts, const int
         }
     }
     // Uniform blocks of whole 16-byte vectors up to 4 KiB into an aligned global buffer:
-    // several sources per pass (alltoall_pull_uniform)
+    // several sources per pass (coll_pull_uniform)
     bool batched = false;
     if (!small && uniform && !needs_staging(rb) && ((uintptr_t)rb & 15) == 0) {
         const uint64_t n = (uint64_t)uniform_rcount * re;
         const uint64_t vpl = (n / 16 + kWave - 1) / kWave;
         if (n != 0 && n % 16 == 0 && vpl <= 4) {
             batched = true;
-            if (vpl == 1) alltoall_pull_uniform<1>((char*)rb, c->slots, me, P, n);
-            else if (vpl == 2) alltoall_pull_uniform<2>((char*)rb, c->slots, me, P, n);
-            else alltoall_pull_uniform<4>((char*)rb, c->slots, me, P, n);
+            coll_pull_blocks((char*)rb, c->slots, me, P, n, (uint64_t)me * n);
         }
     }
     for (int k = 1; k <= P && !small && !batched; k++) {
@@ -558,7 +566,22 @@ __device__ int coll_gatherv(const void* sendbuf, uint64_t sbytes, void* recvbuf,
     if (all) {
         // Allgather(v): pull every rank's block into my buffer
         if (!coll_wait_all(c, seq)) return MPI_ERR_OTHER;
-        for (int k = 0; k < P; k++) {
+        bool batched = false;
+        if (uniform && !needs_staging(rb) && ((uintptr_t)rb & 15) == 0) {
+            // blocks of whole 16-byte vectors up to 4 KiB, every sender's whole (as in a
+            // correct program): several sources per pass (coll_pull_uniform)
+            const uint64_t n = (uint64_t)uniform_rcount * re;
+            const uint64_t vpl = (n / 16 + kWave - 1) / kWave;
+            if (n != 0 && n % 16 == 0 && vpl <= 4) {
+                bool short_src = false;
+                for (int s = lane(); s < P; s += kWave) short_src |= (uint64_t)c->slots[s].a0 < n;
+                if (ballot(short_src) == 0) {
+                    batched = true;
+                    coll_pull_blocks((char*)rb, c->slots, me, P, n, 0);
+                }
+            }
+        }
+        for (int k = 0; k < P && !batched; k++) {
             const int s = (me + k) % P;
             uint64_t doff = uniform ? (uint64_t)s * uniform_rcount * re : (uint64_t)rd[s] * re;
             uint64_t n = uniform ? (uint64_t)uniform_rcount * re : (uint64_t)rc[s] * re;
