@@ -365,11 +365,11 @@ constexpr int64_t kAlltoallLaneBytes = 16;   // blocks up to this size: pulled l
 
 // A uniform Alltoall's (or Allgather's) pull for blocks of whole 16-byte vectors, at most
 // VPL per lane (n <= 1 KiB * VPL), from sources (me + k) mod P, k = 1 .. P: source s's
-// bytes [soff, soff + n) go to rb + s * n.  The send buffers of 64
-// sources at a time are read into the lanes (one gather) and taken out with readlane;
-// each pass then loads the blocks of G = 8 / VPL sources before storing any, so it waits
-// one HBM latency for G blocks where a block-by-block pull waits one per block (two,
-// counting the read of the source's slot).  Alltoall: a sender's block for rank me starts
+// bytes [soff, soff + n) go to rb + s * n.  The send buffers of 64 sources at a time are
+// read into the lanes (one gather) and taken out with readlane; each pass then loads the
+// blocks of G = 8 / VPL sources before storing any, so it waits one HBM latency for G
+// blocks where a block-by-block pull waits one per block (two, counting the read of the
+// source's slot).  Alltoall: a sender's block for rank me starts
 // at soff = me * n (a uniform Alltoall's send and receive blocks have the same bytes);
 // Allgather: soff = 0.  The destination rb + s * n is 16-byte aligned; sources may be
 // unaligned.  Out of line, so its 8 vectors of loads stay out of the rank programs'
