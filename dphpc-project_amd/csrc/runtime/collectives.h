@@ -411,9 +411,52 @@ __device__ __noinline__ void coll_pull_uniform(char* rb, const CollSlot* slots, 
     }
 }
 
-// coll_pull_uniform for n <= 4 KiB of whole 16-byte vectors.
+// The same pull for blocks of at most 32 vectors (n <= 512 B), which would leave most lanes
+// idle: L lanes per source (L = the power of two >= the block's vectors), so a row of 64
+// lanes loads R = 64 / L sources, and a pass loads up to 8 rows before storing any.  A
+// lane's source address comes from the gathered send buffers by a lane shuffle.
+__device__ __noinline__ void coll_pull_spread(char* rb, const CollSlot* slots, int me, int P, uint64_t n,
+                                              uint64_t soff, int lg) {
+    const int l = lane();
+    const int L = 1 << lg, R = kWave >> lg;
+    const int sub = l >> lg, vi = l & (L - 1);
+    const uint64_t nv = n >> 4;
+    const bool active = (uint64_t)vi < nv;
+    const uint64_t col = 16 * (uint64_t)(active ? vi : 0);
+    for (int k0 = 1; k0 <= P; k0 += kWave) {
+        const int nk = P - k0 + 1 < kWave ? P - k0 + 1 : kWave;
+        const uint64_t sb = l < nk ? (uint64_t)(uintptr_t)slots[(me + k0 + l) % P].sbuf : 0;
+        for (int j0 = 0; j0 < nk; j0 += 8 * R) {
+            u32x4 v[8];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                if (j0 + b * R >= nk) break;   // wave-uniform
+                const int j = j0 + b * R + sub;
+                const uint64_t sp = __shfl(sb, j < nk ? j : 0);
+                v[b] = gload16(sp + soff + col);   // every lane loads (source 0 past the end)
+            }
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                if (j0 + b * R >= nk) break;
+                const int j = j0 + b * R + sub;
+                int s = me + k0 + j;   // < 2P where j < nk
+                s -= s >= P ? P : 0;
+                if (j < nk && active) gstore16<false>((uint64_t)(uintptr_t)rb + (uint64_t)s * n + col, v[b]);
+            }
+        }
+    }
+}
+
+// coll_pull_uniform for n <= 4 KiB of whole 16-byte vectors (coll_pull_spread up to 512 B).
 __device__ __forceinline__ void coll_pull_blocks(char* rb, const CollSlot* slots, int me, int P,
                                                  uint64_t n, uint64_t soff) {
+    const uint64_t nv = n / 16;
+    if (nv <= kWave / 2) {
+        int lg = 0;
+        while ((1ull << lg) < nv) lg++;
+        coll_pull_spread(rb, slots, me, P, n, soff, lg);
+        return;
+    }
     const uint64_t vpl = (n / 16 + kWave - 1) / kWave;
     if (vpl == 1) coll_pull_uniform<1>(rb, slots, me, P, n, soff);
     else if (vpl == 2) coll_pull_uniform<2>(rb, slots, me, P, n, soff);

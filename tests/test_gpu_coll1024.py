@@ -96,8 +96,7 @@ def test_collectives_1024_ranks_against_the_oracle(dtype, count, pkg, oracle, to
 @pytest.mark.parametrize("per", [1, 2, 4])
 def test_alltoall_small_blocks_1024_ranks(dtype, per, pkg, oracle, torch_gpu):
     """MPI_Alltoall of 1, 2 and 4 elements per pair (4-32 byte blocks): blocks of up to 16
-    bytes are pulled lane by lane (16 sources per lane), larger ones by the per-source
-    wave copy.  Every destination's whole receive buffer equals the transpose of the
+    bytes are pulled lane by lane (16 sources per lane), 32-byte ones two lanes per source.  Every destination's whole receive buffer equals the transpose of the
     inputs."""
     torch = torch_gpu
     print(f"[gpu] alltoall 1024 ranks dtype={dtype} per={per}", flush=True)
@@ -116,12 +115,15 @@ def test_alltoall_small_blocks_1024_ranks(dtype, per, pkg, oracle, torch_gpu):
             assert bad.size == 0, f"destinations {bad[:8]} differ"
 
 
-@pytest.mark.parametrize("dtype,per", [(MPI_INT, 16), (MPI_INT, 256), (MPI_INT, 512), (MPI_INT, 768),
-                                       (MPI_INT, 1024), (MPI_INT, 1028), (MPI_DOUBLE, 8),
-                                       (MPI_DOUBLE, 3), (MPI_DOUBLE, 128), (MPI_DOUBLE, 512)])
+@pytest.mark.parametrize("dtype,per", [(MPI_INT, 8), (MPI_INT, 12), (MPI_INT, 16), (MPI_INT, 32),
+                                       (MPI_INT, 128), (MPI_INT, 256), (MPI_INT, 512), (MPI_INT, 768),
+                                       (MPI_INT, 1024), (MPI_INT, 1028), (MPI_DOUBLE, 6),
+                                       (MPI_DOUBLE, 8), (MPI_DOUBLE, 3), (MPI_DOUBLE, 128),
+                                       (MPI_DOUBLE, 512)])
 def test_alltoall_blocks_1024_ranks_transpose(dtype, per, pkg, torch_gpu):
-    """MPI_Alltoall of 64 B .. 4 KiB blocks: whole 16-byte vectors up to 4 KiB are pulled
-    several sources per pass (1, 2 or 4 vectors per lane: 8, 4 or 2 sources); 4112-byte and
+    """MPI_Alltoall of 32 B .. 4 KiB blocks: whole 16-byte vectors up to 4 KiB are pulled
+    several sources per pass -- up to 512 B with 2-32 lanes per source (32, 48, 64, 128 and
+    512 B here), above with 1, 2 or 4 vectors per lane (8, 4 or 2 sources); 4112-byte and
     24-byte blocks take the one-source-per-pass copy.  Every destination's receive buffer
     equals the block transpose of the inputs (which the program generated on the device),
     compared on the device; twice per runtime."""
@@ -144,11 +146,12 @@ def test_alltoall_blocks_1024_ranks_transpose(dtype, per, pkg, torch_gpu):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("dtype,count", [(MPI_INT, 16), (MPI_INT, 256), (MPI_INT, 768), (MPI_INT, 1024),
+@pytest.mark.parametrize("dtype,count", [(MPI_INT, 8), (MPI_INT, 16), (MPI_INT, 96), (MPI_INT, 256),
+                                         (MPI_INT, 768), (MPI_INT, 1024),
                                          (MPI_INT, 1028), (MPI_DOUBLE, 3), (MPI_DOUBLE, 512)])
 def test_allgather_1024_ranks(dtype, count, pkg, torch_gpu):
-    """MPI_Allgather of 64 B .. 4 KiB per rank: whole 16-byte vectors up to 4 KiB are pulled
-    several sources per pass, 4112-byte and 24-byte blocks one source at a time.  Every
+    """MPI_Allgather of 32 B .. 4 KiB per rank: whole 16-byte vectors up to 4 KiB are pulled
+    several sources per pass (up to 512 B several sources per row of lanes), 4112-byte and 24-byte blocks one source at a time.  Every
     rank's receive buffer equals all inputs in rank order, compared on the device."""
     torch = torch_gpu
     print(f"[gpu] allgather 1024 ranks dtype={dtype} count={count}", flush=True)
